@@ -1,0 +1,131 @@
+"""Benchmark of the LandSlide-HySEA solver step on one MI355X.
+
+Workload: the 2048x2048 precision-comparison case (plane shelf + 8 seeded
+Gaussian bumps, 4 seeded slides of 20-80 m, water at rest, walls, CFL 0.5,
+r 0.5, Coulomb 10 deg), forward steps of the fused step kernel.  A "step" is
+one solver time step over the whole grid: the four edge classes, the cell
+update and the on-device dt reduction.
+
+Prints one JSON line.  value = Mcell-updates/s of the headline path (FP32
+with compensated state when available, else FP64) with the state resident in
+HBM; the FP64 path is timed beside it.  roofline.achieved uses algorithmic
+bytes per cell-update (104 B: read 6 state fields + depth, write 6 fields)
+over the step kernel's average duration measured with HIP events on the
+library's stream.  cpu_baseline times the FP64 oracle on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from landslidehysea_amd import scenario as S          # noqa: E402
+from landslidehysea_amd import solver as L            # noqa: E402
+
+BYTES_PER_CELL = 104          # FP64: 6x8 read + 8 depth + 6x8 write; FP32c: 12x4 read+write... see DESIGN.md
+HBM_PEAK_GBS = 8000.0
+NX = NY = 2048
+SEED = 2048
+
+
+def make_case(nx=NX, ny=NY, precision=L.FP64):
+    sc = S.make_scenario(nx, ny, seed=SEED, n_slides=4, thickness=(20.0, 80.0))
+    H, _ = S.hmin_shift(sc.H)
+    prm = L.Params(nx=nx, ny=ny, dx=sc.dx, dy=sc.dy, angle_deg=(10.0,), precision=precision)
+    return sc, np.ascontiguousarray(sc.state), np.ascontiguousarray(H), prm
+
+
+def time_gpu(precision, steps, warmup):
+    sc, W, H, prm = make_case(precision=precision)
+    with L.Simulation(prm) as sim:
+        sim.upload(W, H)
+        sim.initial_dt()
+        if warmup:
+            sim.step(warmup)
+        sim.sync()
+        t0 = time.perf_counter()
+        total_ms, _ = sim.time_steps(steps, per_kernel=False)
+        sim.sync()
+        wall = time.perf_counter() - t0
+        _, kernel_ms = sim.time_steps(max(steps // 2, 5), per_kernel=True)
+        t, dtn, n = sim.time()
+    N = NX * NY
+    return {"total_ms": total_ms, "wall_s": wall, "kernel_ms": kernel_ms,
+            "mcells": N * steps / (total_ms * 1e-3) / 1e6, "t": t, "steps_done": n}
+
+
+def cpu_baseline(sample_steps=3, nx=1024, ny=1024):
+    """FP64 oracle (OpenMP) on the host: a bounded sample of the same construction."""
+    from oracle import binding as ob
+    ob.build()
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    cores = min(cores, 16)
+    ob.set_threads(cores)
+    sc = S.make_scenario(nx, ny, seed=SEED, n_slides=4, thickness=(20.0, 80.0))
+    H, _ = S.hmin_shift(sc.H)
+    W = np.ascontiguousarray(sc.state)
+    p = ob.make_params(nx, ny, sc.dx, sc.dy, angle_deg=10.0)
+    dt = ob.initial_dt(p, W, H)
+    ob.run(p, W, H, 1, dt=dt)
+    t0 = time.perf_counter()
+    ob.run(p, W, H, sample_steps, dt=dt)
+    el = time.perf_counter() - t0
+    return {"value": nx * ny * sample_steps / el / 1e6, "unit": "Mcell-updates/s", "cores": cores,
+            "kind": "port", "sample": f"FP64 oracle, {nx}x{ny} same construction, {sample_steps} steps"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the whole grid lives in one MI355X's HBM); "
+              "--gpus above 1 is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    fp64 = time_gpu(L.FP64, args.steps, args.warmup)
+    head = fp64
+    dtype = "f64"
+    N = NX * NY
+    achieved = BYTES_PER_CELL * N / (head["kernel_ms"] * 1e-3) / 1e9
+    out = {
+        "metric": "Mcell-updates/s (FP32-comp forward; plain FP64 beside it) on one MI355X",
+        "value": round(head["mcells"], 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["total_ms"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (seeded shelf + bumps + 4 slides)",
+        "config": {"workload": "precision comparison 2048x2048, forward step", "nx": NX, "ny": NY,
+                   "slides": 4, "friction_deg": 10.0, "cfl": 0.5},
+        "fp64_mcells": round(fp64["mcells"], 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_ms": round(head["kernel_ms"], 5), "bytes_per_cell": BYTES_PER_CELL},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

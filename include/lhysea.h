@@ -1,0 +1,125 @@
+/*
+ * lhysea.h -- C-ABI of the MI355X-native LandSlide-HySEA solver step.
+ *
+ * Plain C types only (pointers, sizes, doubles): no HIP or torch types cross
+ * this boundary.  Every entry point returns an int status (LHYSEA_OK == 0)
+ * and never throws; lhysea_last_error() gives a message for the calling thread.
+ *
+ * Reference interfaces this library replaces:
+ *   lhysea_device_probe   <- extern "C" int comprobarSoporteCUDA()
+ *                            (src/GPU/main.cxx:11, src/GPU/ComprobarSoporteCUDA.cu:5-25)
+ *   lhysea_shallow_water  <- extern "C" int shallowWater(TDatoCluster*, ...)
+ *                            (src/GPU/main.cxx:12-18, src/GPU/ShallowWater.cu:125-486),
+ *                            minus the MPI fields (num_procs, id_hebra, num_voly_otros):
+ *                            the whole grid lives on one GPU.
+ *   lhysea_create/upload/step/download/destroy
+ *                         <- the phases of shallowWater: device allocation and
+ *                            upload (inicializarDatosCuda, src/GPU/ShallowWater.cu:14-110),
+ *                            the step loop (:269-461), copies back (:274-275, :466)
+ *                            and release (liberarSWCuda, :112-122).
+ *   lhysea_initial_dt     <- the initial-dt pass (src/GPU/ShallowWater.cu:236-257)
+ *   lhysea_eta_max        <- d_eta1_maxima download (src/GPU/ShallowWater.cu:466)
+ *   lhysea_set_gauges / lhysea_get_gauges
+ *                         <- the gauge time series (src/GPU/ShallowWater.cu:318-331)
+ *
+ * Threading: a handle owns its device memory and one HIP stream; handles are
+ * not thread-safe but may be used from different threads one after another.
+ */
+#ifndef LHYSEA_H
+#define LHYSEA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes; 1 and 2 keep shallowWater's meaning (src/GPU/ShallowWater.cu:124) */
+#define LHYSEA_OK 0
+#define LHYSEA_ERR_GPU_MEMORY 1
+#define LHYSEA_ERR_HOST_MEMORY 2
+#define LHYSEA_ERR_ARGUMENT 3
+#define LHYSEA_ERR_HIP 4
+#define LHYSEA_ERR_NO_DEVICE 5
+#define LHYSEA_ERR_IO 6
+
+/* arithmetic of the step */
+#define LHYSEA_FP64 0   /* plain FP64 */
+#define LHYSEA_FP32C 1  /* FP32 Riemann solver and fluxes, compensated FP32 state,
+                           FP64 depth, hydrostatic terms, dt and time */
+
+/* step kernels (FP64 only offers both) */
+#define LHYSEA_KERNEL_FUSED 1     /* one tiled launch per step (default) */
+#define LHYSEA_KERNEL_FOURPASS 0  /* one launch per edge class, as the reference */
+
+/* Problem parameters in the reference's internal units (after the optional
+ * normalisation of src/GPU/Problema.cxx:267-287).  Angles in radians. */
+typedef struct lhysea_params {
+    int32_t nx, ny;           /* cells in x (columns) and y (rows); row 0 is "superior" */
+    int32_t friction_law;     /* 0 Coulomb (reference default), 1 Pouliquen */
+    int32_t precision;        /* LHYSEA_FP64 or LHYSEA_FP32C */
+    double dx, dy;            /* ancho_vol, alto_vol */
+    double borde_sup, borde_inf, borde_izq, borde_der;  /* 1 open, 0 / -1 wall */
+    double cfl, r;            /* CFL number, density ratio */
+    double angle1, angle2, angle3, angle4;              /* friction angles (rad) */
+    double mfc, mf0, mfs;     /* interlayer, water-bottom, sediment-bottom friction */
+    double vmax1, vmax2;      /* speed caps (<= 0: off) */
+    double g, eps_h, L, H;    /* gravity, dry threshold, length and depth scales */
+    double peso, beta;        /* 1, 1 in the reference (src/GPU/main.cxx:161) */
+} lhysea_params;
+
+typedef struct lhysea_sim lhysea_sim;
+
+/* 0 ok, 1 no GPU, 2 no usable GPU (comprobarSoporteCUDA's codes) */
+int lhysea_device_probe(void);
+const char *lhysea_last_error(void);
+
+int lhysea_create(const lhysea_params *p, lhysea_sim **out);
+int lhysea_destroy(lhysea_sim *s);
+
+/* Host FP64 planes, each nx*ny row-major: W = h1,q1x,q1y,h2,q2x,q2y (6 planes
+ * back to back), H = depth (positive down, already Hmin-shifted).  Upload also
+ * resets time to 0, eta max to h1+h2-H (src/GPU/Problema.cxx:541-546) and the
+ * step counter. */
+int lhysea_upload(lhysea_sim *s, const double *W, const double *H);
+int lhysea_download(lhysea_sim *s, double *W);
+
+/* Initial dt from the wave-speed pass on the current state; becomes the dt of
+ * the next step.  *dt may be NULL. */
+int lhysea_initial_dt(lhysea_sim *s, double *dt);
+/* Force the dt of the next step. */
+int lhysea_set_dt(lhysea_sim *s, double dt);
+/* Run nsteps steps asynchronously on the handle's stream (no host sync). */
+int lhysea_step(lhysea_sim *s, int nsteps);
+int lhysea_sync(lhysea_sim *s);
+/* time at the current state, dt of the next step, completed steps */
+int lhysea_get_time(lhysea_sim *s, double *t, double *dt_next, int32_t *steps);
+/* Enable (1) or disable (0) the running max of eta = h1 - H
+ * (src/GPU/Volumen_kernel.cu:182-213); val/time are nx*ny each. */
+int lhysea_track_eta_max(lhysea_sim *s, int enable);
+int lhysea_eta_max(lhysea_sim *s, double *val, double *time);
+/* Gauges: eta = h1 + h2 - H sampled at flat cell indices after each step, for
+ * at most max_steps steps counted from the call. */
+int lhysea_set_gauges(lhysea_sim *s, const int32_t *cells, int32_t ng, int32_t max_steps);
+int lhysea_get_gauges(lhysea_sim *s, double *out, int32_t *nrec);
+/* dt used by each step since the last upload (at most max_steps kept). */
+int lhysea_dt_history(lhysea_sim *s, double *out, int32_t max_steps, int32_t *nrec);
+/* LHYSEA_KERNEL_FUSED or LHYSEA_KERNEL_FOURPASS */
+int lhysea_set_kernel(lhysea_sim *s, int32_t mode);
+/* Time nsteps steps with HIP events on the handle's stream: total wall time
+ * of the whole sequence and the average duration of the main step kernel
+ * (bracketed launch by launch in a second pass when kernel_ms is not NULL). */
+int lhysea_time_steps(lhysea_sim *s, int32_t nsteps, double *total_ms, double *kernel_ms);
+
+/* One-shot driver with shallowWater's contract: runs while t < t_end, writes
+ * no files itself (output is the caller's, see L-HySEA.exe), returns the loop
+ * wall time in *elapsed_s.  W is updated in place with the final state; eta
+ * max value/time arrays may be NULL. */
+int lhysea_shallow_water(const lhysea_params *p, double *W, const double *H, double t_end,
+                         double *eta_max, double *eta_max_time, double *elapsed_s, int32_t *steps_done);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LHYSEA_H */

@@ -1,0 +1,50 @@
+// Shared device/host definitions of the LandSlide-HySEA MI355X solver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+// EPSILON = FLT_EPSILON is a model constant of the reference
+// (src/GPU/Constantes.hxx:13); it stays FLT_EPSILON in every precision.
+#define LH_EPS 1.1920928955078125e-07
+#define LH_EPSF 1.1920928955078125e-07f
+#define LH_SQRT2 1.41421356237309504880
+#define LH_BIG 1e30
+
+namespace lhysea {
+
+// Per-run constants, derived on the host from the reference's parameter set
+// (src/GPU/main.cxx:157-164, src/GPU/Problema.cxx:226-296).
+struct Consts {
+    int nx, ny;
+    int friction_law;   // 0 Coulomb, 1 Pouliquen
+    int pad_;
+    double dx, dy, area;
+    double borde_sup, borde_inf, borde_izq, borde_der;
+    double cfl, r, g, eps_h;
+    double mu;          // Coulomb coefficient |tan(angle1)| L/H (src/GPU/Arista_kernel.cu:11-16)
+    double tan1, tan2, tan3, tan4, LoverH;   // Pouliquen law
+    double mfc, mf0, mfs, vmax1, vmax2;
+    double peso, beta;
+};
+
+// Device-side step control block (one per simulation).
+// maxA[] holds max_i A_i bit patterns (A_i >= 0, so the IEEE bit order is the
+// numeric order and an integer atomicMax is an exact float max).
+struct Ctl {
+    unsigned long long maxA_next;   // wave-speed max accumulated by the running step
+    unsigned int blocks_done;       // last-block-done counter
+    int step;                       // steps completed
+    double dt_cur;                  // dt of the running step
+    double t;                       // simulated time at the start of the running step
+    double maxA_last;               // max A of the last completed step (diagnostic)
+};
+
+// dt from the wave-speed max: min_i dt_i with dt_i = 2 CFL area / A_i, or 1e30
+// when A_i < EPSILON (src/GPU/Volumen_kernel.cu:215-229, 255-257); the
+// monotone map makes this the min of the reference's per-cell dt.
+__host__ __device__ inline double dt_from_maxA(double maxA, double cfl, double area)
+{
+    return (maxA < LH_EPS) ? LH_BIG : (2.0 * cfl * area) / maxA;
+}
+
+}  // namespace lhysea

@@ -1,0 +1,465 @@
+// FP64 kernels of the LandSlide-HySEA solver step on MI355X (gfx950).
+//
+// Two step paths share the same edge and cell device code (lhysea_fp64.hpp):
+//  * four-pass: one launch per edge class in the reference's order
+//    (even y, odd y, even x, odd x; src/GPU/ShallowWater.cu:391-427) with
+//    accumulators in HBM, then one cell-update launch.  It is the
+//    structural twin of the reference and a GPU-side cross-check.
+//  * fused: one launch per step.  A workgroup owns a TX x TY tile of cells
+//    whose origin is even in x and y, stages the tile plus a 1-cell halo in
+//    LDS, evaluates the four edge classes in order with LDS accumulators
+//    (the order-dependent positivity limiter reads partial sums, so the
+//    class order is reproduced exactly), applies the cell update and writes
+//    the new state to the other ping-pong buffer.
+// Both end with a wavefront-shuffle max of the per-cell wave-speed sums and
+// one 64-bit atomicMax per workgroup; a one-wave finalize launch turns the
+// max into the lagged dt of the next step on the device (no host sync).
+#pragma clang fp contract(off)
+#include "lhysea_internal.hpp"
+#include "lhysea_fp64.hpp"
+
+namespace lhysea {
+namespace f64 {
+
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Workgroup max of v (all threads call); thread 0 issues one atomicMax.
+template <int NT>
+__device__ __forceinline__ void block_max_atomic(double v, unsigned long long* slot)
+{
+    __shared__ double red[NT / 64];
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int i = 1; i < NT / 64; i++) m = fmax(m, red[i]);
+        if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__device__ __forceinline__ double step_dt(const Ctl* ctl, const double* dt_seq)
+{
+    return dt_seq ? dt_seq[ctl->step] : ctl->dt_cur;
+}
+
+__device__ __forceinline__ void load6(const double* __restrict__ W, size_t N, size_t c, double* o)
+{
+#pragma unroll
+    for (int f = 0; f < 6; f++) o[f] = W[f * N + c];
+}
+
+// ---------------------------------------------------------------- initial dt
+// Per-cell gather of the four edges' wave-speed terms in class order, so the
+// sum rounds like the reference's class-by-class accumulation
+// (src/GPU/ShallowWater.cu:238-251).
+__global__ void __launch_bounds__(256) k_dt0(Consts k, const double* __restrict__ W, Ctl* ctl)
+{
+    const int nx = k.nx, ny = k.ny;
+    const size_t N = (size_t)nx * ny;
+    const long cidx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double A = 0.0;
+    if (cidx < (long)N) {
+        const int i = (int)(cidx % nx), j = (int)(cidx / nx);
+        double Wc[6], Wo[6];
+        load6(W, N, cidx, Wc);
+        double v[4];
+        // y-edges: top je = j, bottom je = j + 1; x-edges: left ie = i, right ie = i + 1
+        for (int s = 0; s < 2; s++) {
+            const int je = j + s;
+            double val;
+            if (je == 0 || je == ny) {
+                const double bd = (je == 0) ? k.borde_sup : k.borde_inf;
+                for (int f = 0; f < 6; f++) Wo[f] = Wc[f];
+                Wo[2] *= bd; Wo[5] *= bd;
+                val = edge_dt0(k, Wc, Wo, 0.0, (je == 0) ? -1.0 : 1.0, k.dx);
+            } else if (s == 0) {
+                load6(W, N, cidx - nx, Wo);
+                val = edge_dt0(k, Wo, Wc, 0.0, 1.0, k.dx);
+            } else {
+                load6(W, N, cidx + nx, Wo);
+                val = edge_dt0(k, Wc, Wo, 0.0, 1.0, k.dx);
+            }
+            v[(je & 1)] = val;
+        }
+        for (int s = 0; s < 2; s++) {
+            const int ie = i + s;
+            double val;
+            if (ie == 0 || ie == nx) {
+                const double bd = (ie == 0) ? k.borde_izq : k.borde_der;
+                for (int f = 0; f < 6; f++) Wo[f] = Wc[f];
+                Wo[1] *= bd; Wo[4] *= bd;
+                val = edge_dt0(k, Wc, Wo, (ie == 0) ? -1.0 : 1.0, 0.0, k.dy);
+            } else if (s == 0) {
+                load6(W, N, cidx - 1, Wo);
+                val = edge_dt0(k, Wo, Wc, 1.0, 0.0, k.dy);
+            } else {
+                load6(W, N, cidx + 1, Wo);
+                val = edge_dt0(k, Wc, Wo, 1.0, 0.0, k.dy);
+            }
+            v[2 + (ie & 1)] = val;
+        }
+        A = ((v[0] + v[1]) + v[2]) + v[3];
+    }
+    block_max_atomic<256>(A, &ctl->maxA_next);
+}
+
+__global__ void k_dt0_finalize(Consts k, Ctl* ctl)
+{
+    if (threadIdx.x == 0) {
+        double maxA = __longlong_as_double((long long)ctl->maxA_next);
+        ctl->dt_cur = dt_from_maxA(maxA, k.cfl, k.area);
+        ctl->maxA_last = maxA;
+        ctl->maxA_next = 0ull;
+    }
+}
+
+// Step finalize: record dt, sample gauges on the new state, advance time and
+// derive the next (lagged) dt from this step's wave-speed max
+// (src/GPU/ShallowWater.cu:431-442).
+__global__ void k_finalize(Consts k, const double* __restrict__ Wn, const double* __restrict__ H, StepArgs a)
+{
+    Ctl* ctl = a.ctl;
+    const size_t N = (size_t)k.nx * k.ny;
+    const int s = ctl->step;
+    const double dt = step_dt(ctl, a.dt_seq);
+    if (a.gseries && s < a.gcap)
+        for (int g = threadIdx.x; g < a.ng; g += blockDim.x) {
+            const size_t c = (size_t)a.gauges[g];
+            a.gseries[(size_t)s * a.ng + g] = Wn[c] + Wn[3 * N + c] - H[c];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (a.dt_hist && s < a.hcap) a.dt_hist[s] = dt;
+        double maxA = __longlong_as_double((long long)ctl->maxA_next);
+        ctl->t = ctl->t + dt;
+        ctl->dt_cur = dt_from_maxA(maxA, k.cfl, k.area);
+        ctl->maxA_last = maxA;
+        ctl->maxA_next = 0ull;
+        ctl->step = s + 1;
+    }
+}
+
+// ------------------------------------------------------------ four-pass path
+// One thread per edge of one class (src/GPU/Arista_kernel.cu:651-949).
+__global__ void __launch_bounds__(256) k_edges(Consts k, const double* __restrict__ W,
+                                               const double* __restrict__ H, double* __restrict__ acc,
+                                               const Ctl* ctl, const double* dt_seq, int cls)
+{
+    const int nx = k.nx, ny = k.ny;
+    const size_t N = (size_t)nx * ny;
+    const double dt = step_dt(ctl, dt_seq);
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double W0[6], W1[6];
+    size_t c0, c1 = 0;
+    double n1x, n1y, len, H0, H1;
+    bool interior;
+    if (cls < 2) {
+        const int nrow = (ny - cls) / 2 + 1;
+        if (e >= (long)nrow * nx) return;
+        const int je = cls + 2 * (int)(e / nx), i = (int)(e % nx);
+        len = k.dx;
+        n1x = 0.0;
+        if (je == 0 || je == ny) {
+            c0 = (je == 0) ? (size_t)i : (size_t)(ny - 1) * nx + i;
+            const double bd = (je == 0) ? k.borde_sup : k.borde_inf;
+            load6(W, N, c0, W0);
+            for (int f = 0; f < 6; f++) W1[f] = W0[f];
+            W1[2] *= bd; W1[5] *= bd;
+            n1y = (je == 0) ? -1.0 : 1.0;
+            H0 = H1 = H[c0];
+            interior = false;
+        } else {
+            c0 = (size_t)(je - 1) * nx + i;
+            c1 = c0 + nx;
+            load6(W, N, c0, W0);
+            load6(W, N, c1, W1);
+            n1y = 1.0;
+            H0 = H[c0];
+            H1 = H[c1];
+            interior = true;
+        }
+    } else {
+        const int par = cls - 2, ncol = (nx - par) / 2 + 1;
+        if (e >= (long)ncol * ny) return;
+        const int j = (int)(e / ncol), ie = par + 2 * (int)(e % ncol);
+        len = k.dy;
+        n1y = 0.0;
+        if (ie == 0 || ie == nx) {
+            c0 = (ie == 0) ? (size_t)j * nx : (size_t)j * nx + nx - 1;
+            const double bd = (ie == 0) ? k.borde_izq : k.borde_der;
+            load6(W, N, c0, W0);
+            for (int f = 0; f < 6; f++) W1[f] = W0[f];
+            W1[1] *= bd; W1[4] *= bd;
+            n1x = (ie == 0) ? -1.0 : 1.0;
+            H0 = H1 = H[c0];
+            interior = false;
+        } else {
+            c0 = (size_t)j * nx + ie - 1;
+            c1 = c0 + 1;
+            load6(W, N, c0, W0);
+            load6(W, N, c1, W1);
+            n1x = 1.0;
+            H0 = H[c0];
+            H1 = H[c1];
+            interior = true;
+        }
+    }
+    const double a0h1 = acc[c0], a0h2 = acc[3 * N + c0];
+    const double a1h1 = interior ? acc[c1] : 0.0, a1h2 = interior ? acc[3 * N + c1] : 0.0;
+    EdgeFlux F;
+    if (edge(k, W0, W1, H0, H1, n1x, n1y, len, dt, a0h1, a0h2, a1h1, a1h2, interior, F)) {
+        for (int f = 0; f < 6; f++) acc[f * N + c0] -= F.Fm[f];
+        acc[6 * N + c0] += F.lam;
+        if (interior) {
+            for (int f = 0; f < 6; f++) acc[f * N + c1] -= F.Fp[f];
+            acc[6 * N + c1] += F.lam;
+        }
+    }
+}
+
+// Cell update of the four-pass path; clears the accumulators for the next step.
+__global__ void __launch_bounds__(256) k_update(Consts k, const double* __restrict__ W, double* __restrict__ Wn,
+                                                const double* __restrict__ H, double* __restrict__ acc, StepArgs a)
+{
+    const size_t N = (size_t)k.nx * k.ny;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double A = 0.0;
+    if (c < (long)N) {
+        const double dt = step_dt(a.ctl, a.dt_seq);
+        double o[6], ac[6], n[6];
+        load6(W, N, c, o);
+        for (int f = 0; f < 6; f++) ac[f] = acc[f * N + c];
+        A = acc[6 * N + c];
+        if (a.emax) {
+            const double v = o[0] - H[c];
+            if (v > a.emax[2 * c]) {
+                a.emax[2 * c] = v;
+                a.emax[2 * c + 1] = a.ctl->t;
+            }
+        }
+        cell_update(k, o, ac, dt, n);
+        for (int f = 0; f < 6; f++) Wn[f * N + c] = n[f];
+        for (int f = 0; f < 7; f++) acc[f * N + c] = 0.0;
+    }
+    block_max_atomic<256>(A, &a.ctl->maxA_next);
+}
+
+// ----------------------------------------------------------------- fused path
+// Tile of TX x TY cells (both even), NT threads.  LDS planes:
+//   sW[f][r][c]  f = h1,q1x,q1y,h2,q2x,q2y,H ; r = tile row + 1 ; c = tile col + 1
+//   sA[f][r][c]  f = dh1..dq2y, A            ; r = tile row     ; c = tile col + 1
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_fused(Consts k, const double* __restrict__ W, double* __restrict__ Wn,
+                                              const double* __restrict__ H, StepArgs a, int tiles_x, int tiles_y)
+{
+    constexpr int CW = TX + 2;              // staged columns
+    constexpr int RW = TY + 2;              // staged rows
+    __shared__ double sW[7][RW][CW];
+    __shared__ double sA[7][TY][CW];
+
+    const int nx = k.nx, ny = k.ny;
+    const size_t N = (size_t)nx * ny;
+    // XCD-aware tile order: consecutive workgroups go round-robin over the 8
+    // XCDs, so give each XCD a contiguous band of tiles (shared halos stay in
+    // that XCD's L2).
+    const int nb = tiles_x * tiles_y;
+    int b;
+    {
+        const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+        const int q = nb >> 3, rem = nb & 7;
+        b = xcd * q + (xcd < rem ? xcd : rem) + slot;
+    }
+    const int tx = b % tiles_x, ty = b / tiles_x;
+    const int i0 = tx * TX, j0 = ty * TY;
+    const double dt = step_dt(a.ctl, a.dt_seq);
+    const int tid = threadIdx.x;
+
+    // stage state + depth with a 1-cell halo; zero the accumulators
+    for (int e = tid; e < RW * CW; e += NT) {
+        const int r = e / CW, c = e % CW;
+        const int gj = j0 + r - 1, gi = i0 + c - 1;
+        if (gj >= 0 && gj < ny && gi >= 0 && gi < nx) {
+            const size_t g = (size_t)gj * nx + gi;
+#pragma unroll
+            for (int f = 0; f < 6; f++) sW[f][r][c] = W[f * N + g];
+            sW[6][r][c] = H[g];
+        }
+    }
+    for (int e = tid; e < TY * CW; e += NT) {
+        const int r = e / CW, c = e % CW;
+#pragma unroll
+        for (int f = 0; f < 7; f++) sA[f][r][c] = 0.0;
+    }
+    __syncthreads();
+
+    // y-edge classes C1 (even) then C2 (odd), on staged columns -1..TX
+#pragma unroll 1
+    for (int par = 0; par < 2; par++) {
+        constexpr int CWc = CW;
+        const int nrow = (par == 0) ? (TY / 2 + 1) : (TY / 2);
+        for (int e = tid; e < nrow * CWc; e += NT) {
+            const int c = e % CWc, jl = par + 2 * (e / CWc);   // tile-local edge row
+            const int gi = i0 + c - 1, je = j0 + jl;
+            if (gi < 0 || gi >= nx || je > ny) continue;
+            double W0[6], W1[6], H0, H1, a1h1 = 0.0, a1h2 = 0.0;
+            int r0;                      // tile row of cell 0
+            bool interior;
+            double n1y;
+            if (je == 0 || je == ny) {
+                r0 = (je == 0) ? jl : jl - 1;
+                const double bd = (je == 0) ? k.borde_sup : k.borde_inf;
+#pragma unroll
+                for (int f = 0; f < 6; f++) { W0[f] = sW[f][r0 + 1][c]; W1[f] = W0[f]; }
+                W1[2] *= bd; W1[5] *= bd;
+                H0 = H1 = sW[6][r0 + 1][c];
+                n1y = (je == 0) ? -1.0 : 1.0;
+                interior = false;
+            } else {
+                r0 = jl - 1;
+#pragma unroll
+                for (int f = 0; f < 6; f++) { W0[f] = sW[f][r0 + 1][c]; W1[f] = sW[f][r0 + 2][c]; }
+                H0 = sW[6][r0 + 1][c];
+                H1 = sW[6][r0 + 2][c];
+                n1y = 1.0;
+                interior = true;
+            }
+            const bool has0 = (r0 >= 0 && r0 < TY), has1 = interior && (r0 + 1 < TY);
+            const double a0h1 = has0 ? sA[0][r0][c] : 0.0, a0h2 = has0 ? sA[3][r0][c] : 0.0;
+            if (has1) { a1h1 = sA[0][r0 + 1][c]; a1h2 = sA[3][r0 + 1][c]; }
+            EdgeFlux F;
+            if (edge(k, W0, W1, H0, H1, 0.0, n1y, k.dx, dt, a0h1, a0h2, a1h1, a1h2, interior, F)) {
+                if (has0) {
+#pragma unroll
+                    for (int f = 0; f < 6; f++) sA[f][r0][c] -= F.Fm[f];
+                    sA[6][r0][c] += F.lam;
+                }
+                if (has1) {
+#pragma unroll
+                    for (int f = 0; f < 6; f++) sA[f][r0 + 1][c] -= F.Fp[f];
+                    sA[6][r0 + 1][c] += F.lam;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // x-edge classes C3 (even) then C4 (odd), on tile rows 0..TY-1
+#pragma unroll 1
+    for (int par = 0; par < 2; par++) {
+        const int ncol = (par == 0) ? (TX / 2 + 1) : (TX / 2);
+        for (int e = tid; e < ncol * TY; e += NT) {
+            const int r = e / ncol, il = par + 2 * (e % ncol);   // tile-local edge column
+            const int gj = j0 + r, ie = i0 + il;
+            if (gj >= ny || ie > nx) continue;
+            double W0[6], W1[6], H0, H1, a1h1 = 0.0, a1h2 = 0.0;
+            int c0;                      // staged column of cell 0
+            bool interior;
+            double n1x;
+            if (ie == 0 || ie == nx) {
+                c0 = (ie == 0) ? il + 1 : il;
+                const double bd = (ie == 0) ? k.borde_izq : k.borde_der;
+#pragma unroll
+                for (int f = 0; f < 6; f++) { W0[f] = sW[f][r + 1][c0]; W1[f] = W0[f]; }
+                W1[1] *= bd; W1[4] *= bd;
+                H0 = H1 = sW[6][r + 1][c0];
+                n1x = (ie == 0) ? -1.0 : 1.0;
+                interior = false;
+            } else {
+                c0 = il;
+#pragma unroll
+                for (int f = 0; f < 6; f++) { W0[f] = sW[f][r + 1][c0]; W1[f] = sW[f][r + 1][c0 + 1]; }
+                H0 = sW[6][r + 1][c0];
+                H1 = sW[6][r + 1][c0 + 1];
+                n1x = 1.0;
+                interior = true;
+            }
+            const double a0h1 = sA[0][r][c0], a0h2 = sA[3][r][c0];
+            if (interior) { a1h1 = sA[0][r][c0 + 1]; a1h2 = sA[3][r][c0 + 1]; }
+            EdgeFlux F;
+            if (edge(k, W0, W1, H0, H1, n1x, 0.0, k.dy, dt, a0h1, a0h2, a1h1, a1h2, interior, F)) {
+#pragma unroll
+                for (int f = 0; f < 6; f++) sA[f][r][c0] -= F.Fm[f];
+                sA[6][r][c0] += F.lam;
+                if (interior) {
+#pragma unroll
+                    for (int f = 0; f < 6; f++) sA[f][r][c0 + 1] -= F.Fp[f];
+                    sA[6][r][c0 + 1] += F.lam;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // cell update of the tile's own cells
+    double Amax = 0.0;
+    const double tnow = a.ctl->t;
+    for (int e = tid; e < TX * TY; e += NT) {
+        const int r = e / TX, c = e % TX;
+        const int gj = j0 + r, gi = i0 + c;
+        if (gj >= ny || gi >= nx) continue;
+        const size_t g = (size_t)gj * nx + gi;
+        double o[6], ac[6], n[6];
+#pragma unroll
+        for (int f = 0; f < 6; f++) { o[f] = sW[f][r + 1][c + 1]; ac[f] = sA[f][r][c + 1]; }
+        Amax = fmax(Amax, sA[6][r][c + 1]);
+        if (a.emax) {
+            const double v = o[0] - sW[6][r + 1][c + 1];
+            if (v > a.emax[2 * g]) {
+                a.emax[2 * g] = v;
+                a.emax[2 * g + 1] = tnow;
+            }
+        }
+        cell_update(k, o, ac, dt, n);
+#pragma unroll
+        for (int f = 0; f < 6; f++) Wn[f * N + g] = n[f];
+    }
+    block_max_atomic<NT>(Amax, &a.ctl->maxA_next);
+}
+
+constexpr int FTX = 30, FTY = 14, FNT = 256;
+
+}  // namespace f64
+
+// ------------------------------------------------------------------ launchers
+hipError_t f64_initial_dt(const Consts& k, const double* W, Ctl* ctl, hipStream_t s)
+{
+    const size_t N = (size_t)k.nx * k.ny;
+    hipLaunchKernelGGL(f64::k_dt0, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, k, W, ctl);
+    hipLaunchKernelGGL(f64::k_dt0_finalize, dim3(1), dim3(64), 0, s, k, ctl);
+    return hipGetLastError();
+}
+
+hipError_t f64_step_fourpass(const Consts& k, const double* W, double* Wn, const double* H, double* acc,
+                             const StepArgs& a, hipStream_t s)
+{
+    const size_t N = (size_t)k.nx * k.ny;
+    for (int cls = 0; cls < 4; cls++) {
+        long ne;
+        if (cls < 2) ne = (long)((k.ny - cls) / 2 + 1) * k.nx;
+        else ne = (long)((k.nx - (cls - 2)) / 2 + 1) * k.ny;
+        hipLaunchKernelGGL(f64::k_edges, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, k, W, H, acc,
+                           a.ctl, a.dt_seq, cls);
+    }
+    hipLaunchKernelGGL(f64::k_update, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, k, W, Wn, H, acc, a);
+    hipLaunchKernelGGL(f64::k_finalize, dim3(1), dim3(64), 0, s, k, Wn, H, a);
+    return hipGetLastError();
+}
+
+hipError_t f64_step_fused(const Consts& k, const double* W, double* Wn, const double* H, const StepArgs& a,
+                          hipStream_t s)
+{
+    const int tiles_x = (k.nx + f64::FTX - 1) / f64::FTX;
+    const int tiles_y = (k.ny + f64::FTY - 1) / f64::FTY;
+    hipLaunchKernelGGL((f64::k_fused<f64::FTX, f64::FTY, f64::FNT>), dim3((unsigned)(tiles_x * tiles_y)),
+                       dim3(f64::FNT), 0, s, k, W, Wn, H, a, tiles_x, tiles_y);
+    hipLaunchKernelGGL(f64::k_finalize, dim3(1), dim3(64), 0, s, k, Wn, H, a);
+    return hipGetLastError();
+}
+
+}  // namespace lhysea

@@ -1,0 +1,264 @@
+"""Host-side mirror of the reference's solver interface over the C-ABI.
+
+The reference drives one simulation through ``shallowWater(...)``
+(src/GPU/ShallowWater.cu:125-486); this module exposes the same phases over
+``liblhysea_hip.so`` (declared in include/lhysea.h): create, upload, initial
+dt, step, download, eta max, gauges and the one-shot driver.  The HIP library
+is mandatory: importing works without it, but every call raises
+``NativeLibraryMissing`` when the built library is absent.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblhysea_hip.so")
+
+FP64 = 0
+FP32C = 1
+KERNEL_FOURPASS = 0
+KERNEL_FUSED = 1
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+class LhyseaError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"lhysea error {code}: {msg}")
+        self.code = code
+
+
+class CParams(C.Structure):
+    """Mirror of ``lhysea_params`` (include/lhysea.h)."""
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("friction_law", C.c_int32), ("precision", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double),
+        ("borde_sup", C.c_double), ("borde_inf", C.c_double),
+        ("borde_izq", C.c_double), ("borde_der", C.c_double),
+        ("cfl", C.c_double), ("r", C.c_double),
+        ("angle1", C.c_double), ("angle2", C.c_double), ("angle3", C.c_double), ("angle4", C.c_double),
+        ("mfc", C.c_double), ("mf0", C.c_double), ("mfs", C.c_double),
+        ("vmax1", C.c_double), ("vmax2", C.c_double),
+        ("g", C.c_double), ("eps_h", C.c_double), ("L", C.c_double), ("H", C.c_double),
+        ("peso", C.c_double), ("beta", C.c_double),
+    ]
+
+
+@dataclass
+class Params:
+    """The reference's parameter set in internal units (src/GPU/Problema.cxx:226-296).
+
+    ``angle_deg`` is the Coulomb friction angle (one value) or the four
+    Pouliquen angles; ``borde`` is (sup, inf, izq, der)."""
+    nx: int
+    ny: int
+    dx: float
+    dy: float
+    borde: tuple = (-1.0, -1.0, -1.0, -1.0)
+    cfl: float = 0.5
+    r: float = 0.5
+    angle_deg: tuple = (10.0,)
+    mfc: float = 0.08
+    mf0: float = 0.03
+    mfs: float = 0.03
+    vmax1: float = 0.0
+    vmax2: float = 0.0
+    g: float = 9.81
+    eps_h: float = 5e-3
+    L: float = 1.0
+    H: float = 1.0
+    friction_law: int = 0
+    precision: int = FP64
+    peso: float = 1.0
+    beta: float = 1.0
+
+    def to_c(self) -> CParams:
+        p = CParams()
+        p.nx, p.ny, p.friction_law, p.precision = self.nx, self.ny, self.friction_law, self.precision
+        p.dx, p.dy = self.dx, self.dy
+        p.borde_sup, p.borde_inf, p.borde_izq, p.borde_der = self.borde
+        p.cfl, p.r = self.cfl, self.r
+        a = tuple(self.angle_deg)
+        if len(a) == 1:
+            a = a * 4
+        p.angle1, p.angle2, p.angle3, p.angle4 = [x * math.pi / 180.0 for x in a]
+        p.mfc, p.mf0, p.mfs, p.vmax1, p.vmax2 = self.mfc, self.mf0, self.mfs, self.vmax1, self.vmax2
+        p.g, p.eps_h, p.L, p.H = self.g, self.eps_h, self.L, self.H
+        p.peso, p.beta = self.peso, self.beta
+        return p
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lib = None
+
+# every entry point of include/lhysea.h with its ctypes signature
+SIGNATURES = {
+    "lhysea_device_probe": (C.c_int, []),
+    "lhysea_last_error": (C.c_char_p, []),
+    "lhysea_create": (C.c_int, [C.POINTER(CParams), C.POINTER(C.c_void_p)]),
+    "lhysea_destroy": (C.c_int, [C.c_void_p]),
+    "lhysea_upload": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "lhysea_download": (C.c_int, [C.c_void_p, _dp]),
+    "lhysea_initial_dt": (C.c_int, [C.c_void_p, _dp]),
+    "lhysea_set_dt": (C.c_int, [C.c_void_p, C.c_double]),
+    "lhysea_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "lhysea_sync": (C.c_int, [C.c_void_p]),
+    "lhysea_get_time": (C.c_int, [C.c_void_p, _dp, _dp, _ip]),
+    "lhysea_track_eta_max": (C.c_int, [C.c_void_p, C.c_int]),
+    "lhysea_eta_max": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "lhysea_set_gauges": (C.c_int, [C.c_void_p, _ip, C.c_int32, C.c_int32]),
+    "lhysea_get_gauges": (C.c_int, [C.c_void_p, _dp, _ip]),
+    "lhysea_dt_history": (C.c_int, [C.c_void_p, _dp, C.c_int32, _ip]),
+    "lhysea_set_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lhysea_time_steps": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
+    "lhysea_shallow_water": (C.c_int, [C.POINTER(CParams), _dp, _dp, C.c_double, _dp, _dp, _dp, _ip]),
+}
+
+
+def load_library(path: str = LIB_PATH):
+    """Load liblhysea_hip.so (raises NativeLibraryMissing when it is not built)."""
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise NativeLibraryMissing(f"{path} is not built; run __graft_entry__.build() or "
+                                   "python landslidehysea_amd/build.py")
+    L = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path == LIB_PATH:
+        _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise LhyseaError(rc, load_library().lhysea_last_error().decode())
+
+
+def _dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def device_probe() -> int:
+    return load_library().lhysea_device_probe()
+
+
+class Simulation:
+    """One simulation on the GPU (a handle of the C-ABI)."""
+
+    def __init__(self, params: Params, kernel: int = KERNEL_FUSED):
+        self.params = params
+        self.lib = load_library()
+        self._cp = params.to_c()
+        h = C.c_void_p()
+        _check(self.lib.lhysea_create(C.byref(self._cp), C.byref(h)))
+        self.h = h
+        self.N = params.nx * params.ny
+        if kernel != KERNEL_FUSED:
+            _check(self.lib.lhysea_set_kernel(self.h, kernel))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lhysea_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, W: np.ndarray, H: np.ndarray):
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(6, -1)
+        H = np.ascontiguousarray(H, dtype=np.float64).reshape(-1)
+        assert W.shape[1] == self.N and H.size == self.N
+        _check(self.lib.lhysea_upload(self.h, _dptr(W), _dptr(H)))
+
+    def download(self) -> np.ndarray:
+        W = np.zeros((6, self.params.ny, self.params.nx))
+        _check(self.lib.lhysea_download(self.h, _dptr(W)))
+        return W
+
+    def initial_dt(self) -> float:
+        d = C.c_double()
+        _check(self.lib.lhysea_initial_dt(self.h, C.byref(d)))
+        return d.value
+
+    def set_dt(self, dt: float):
+        _check(self.lib.lhysea_set_dt(self.h, dt))
+
+    def step(self, n: int = 1):
+        _check(self.lib.lhysea_step(self.h, n))
+
+    def sync(self):
+        _check(self.lib.lhysea_sync(self.h))
+
+    def time(self):
+        t, d, n = C.c_double(), C.c_double(), C.c_int32()
+        _check(self.lib.lhysea_get_time(self.h, C.byref(t), C.byref(d), C.byref(n)))
+        return t.value, d.value, n.value
+
+    def track_eta_max(self, on: bool = True):
+        _check(self.lib.lhysea_track_eta_max(self.h, 1 if on else 0))
+
+    def eta_max(self):
+        v = np.zeros((self.params.ny, self.params.nx))
+        t = np.zeros_like(v)
+        _check(self.lib.lhysea_eta_max(self.h, _dptr(v), _dptr(t)))
+        return v, t
+
+    def set_gauges(self, cells, max_steps: int):
+        c = np.ascontiguousarray(cells, dtype=np.int32)
+        _check(self.lib.lhysea_set_gauges(self.h, c.ctypes.data_as(_ip), len(c), max_steps))
+        self._ng = len(c)
+        self._gcap = max_steps
+
+    def gauges(self) -> np.ndarray:
+        out = np.zeros((self._gcap, self._ng))
+        n = C.c_int32()
+        _check(self.lib.lhysea_get_gauges(self.h, _dptr(out), C.byref(n)))
+        return out[: n.value]
+
+    def dt_history(self, max_steps: int) -> np.ndarray:
+        out = np.zeros(max_steps)
+        n = C.c_int32()
+        _check(self.lib.lhysea_dt_history(self.h, _dptr(out), max_steps, C.byref(n)))
+        return out[: n.value]
+
+    def time_steps(self, nsteps: int, per_kernel: bool = True):
+        tot, ker = C.c_double(), C.c_double()
+        _check(self.lib.lhysea_time_steps(self.h, nsteps, C.byref(tot), C.byref(ker) if per_kernel else None))
+        return tot.value, (ker.value if per_kernel else None)
+
+
+def shallow_water(params: Params, W: np.ndarray, H: np.ndarray, t_end: float):
+    """One-shot driver with shallowWater's contract; returns (W, eta_max, eta_max_time, seconds, steps)."""
+    lib = load_library()
+    cp = params.to_c()
+    W = np.ascontiguousarray(W, dtype=np.float64).copy()
+    H = np.ascontiguousarray(H, dtype=np.float64)
+    em = np.zeros(params.nx * params.ny)
+    et = np.zeros_like(em)
+    el = C.c_double()
+    n = C.c_int32()
+    _check(lib.lhysea_shallow_water(C.byref(cp), _dptr(W), _dptr(H), t_end, _dptr(em), _dptr(et),
+                                    C.byref(el), C.byref(n)))
+    return W, em.reshape(params.ny, params.nx), et.reshape(params.ny, params.nx), el.value, n.value
