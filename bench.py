@@ -91,15 +91,22 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=["both", "fp32c", "fp64"], default="both",
+                    help="paths to time (profiling runs time one)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the whole grid lives in one MI355X's HBM); "
               "--gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
 
-    fp64 = time_gpu(L.FP64, args.steps, args.warmup)
-    head = fp64
-    dtype = "f64"
+    fp32c = fp64 = None
+    if args.precision in ("both", "fp32c"):
+        fp32c = time_gpu(L.FP32C, args.steps, args.warmup)
+    if args.precision in ("both", "fp64"):
+        fp64 = time_gpu(L.FP64, max(args.steps // 4, 10) if fp32c else args.steps,
+                        max(args.warmup // 4, 2) if fp32c else args.warmup)
+    head = fp32c if fp32c else fp64
+    dtype = "f32 (compensated state, f64 depth/hydrostatics/dt)" if fp32c else "f64"
     N = NX * NY
     achieved = BYTES_PER_CELL * N / (head["kernel_ms"] * 1e-3) / 1e9
     out = {
@@ -117,7 +124,9 @@ def main():
         "data": "synthetic (seeded shelf + bumps + 4 slides)",
         "config": {"workload": "precision comparison 2048x2048, forward step", "nx": NX, "ny": NY,
                    "slides": 4, "friction_deg": 10.0, "cfl": 0.5},
-        "fp64_mcells": round(fp64["mcells"], 2),
+        "fp64_mcells": round(fp64["mcells"], 2) if fp64 else None,
+        "speedup_vs_fp64": round(fp32c["mcells"] / fp64["mcells"], 3) if (fp32c and fp64) else None,
+        "fp64_kernel_ms": round(fp64["kernel_ms"], 5) if fp64 else None,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel_ms": round(head["kernel_ms"], 5), "bytes_per_cell": BYTES_PER_CELL},

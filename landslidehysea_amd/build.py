@@ -25,6 +25,24 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+HOST_SOURCES = ["host/lhysea_main.cpp", "host/lhysea_problem.cpp", "host/lhysea_netcdf.cpp"]
+EXE = os.path.join(HERE, "L-HySEA.exe")
+
+
+def build_cli(verbose: bool = False) -> str:
+    """Host front end L-HySEA.exe (plain C++), linked against liblhysea_hip.so."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall"] + [os.path.join(CSRC, s) for s in HOST_SOURCES] + \
+          ["-o", EXE + ".tmp", "-L" + HERE, "-l:liblhysea_hip.so", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build of L-HySEA.exe failed")
+    os.replace(EXE + ".tmp", EXE)
+    return EXE
+
+
 def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     cmd = [hipcc()] + HIP_FLAGS + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", tmp]
@@ -35,6 +53,7 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc build of liblhysea_hip.so failed")
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
 
 

@@ -108,9 +108,11 @@ static StepArgs step_args(lhysea_sim* s)
     return a;
 }
 
-static int launch_steps(lhysea_sim* s, int nsteps)
+static int launch_steps(lhysea_sim* s, int nsteps, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
 {
     StepArgs a = step_args(s);
+    a.ev0 = ev0;
+    a.ev1 = ev1;
     if (s->gseries) {
         // gauge rows are indexed by ctl->step; offset the base so row 0 is the
         // first step after lhysea_set_gauges
@@ -158,6 +160,8 @@ int lhysea_create(const lhysea_params* p, lhysea_sim** out)
         return fail(LHYSEA_ERR_ARGUMENT, "unknown precision");
     if (!(p->dx > 0) || !(p->dy > 0) || !(p->eps_h > 0) || !(p->H > 0) || !(p->peso > 0))
         return fail(LHYSEA_ERR_ARGUMENT, "dx, dy, eps_h, H and peso must be positive");
+    if (p->precision == LHYSEA_FP32C && (p->friction_law != 0 || p->peso != 1.0 || p->beta != 1.0))
+        return fail(LHYSEA_ERR_ARGUMENT, "the FP32C path implements the Coulomb law with peso = beta = 1");
     if (lhysea_device_probe() != 0) return fail(LHYSEA_ERR_NO_DEVICE, "no usable HIP device");
     lhysea_sim* s = new (std::nothrow) lhysea_sim;
     if (!s) return fail(LHYSEA_ERR_HOST_MEMORY, "host allocation failed");
@@ -429,14 +433,20 @@ int lhysea_time_steps(lhysea_sim* s, int32_t nsteps, double* total_ms, double* k
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     if (total_ms) *total_ms = ms;
     if (kernel_ms) {
-        // per-launch bracketing of the main step kernel: events around each
-        // step; the step's finalize launch is a single wave and is included
+        // events recorded by the launcher immediately around the main step
+        // kernel (fused kernel, or the cell-update kernel of the four-pass path
+        // is not bracketed: there the whole step is bracketed instead)
         double sum = 0.0;
         for (int n = 0; n < nsteps; n++) {
-            HIPCHK(hipEventRecord(e0, s->stream));
-            rc = launch_steps(s, 1);
-            if (rc) return rc;
-            HIPCHK(hipEventRecord(e1, s->stream));
+            if (s->mode == MODE_FUSED) {
+                rc = launch_steps(s, 1, e0, e1);
+                if (rc) return rc;
+            } else {
+                HIPCHK(hipEventRecord(e0, s->stream));
+                rc = launch_steps(s, 1);
+                if (rc) return rc;
+                HIPCHK(hipEventRecord(e1, s->stream));
+            }
             HIPCHK(hipEventSynchronize(e1));
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
             sum += ms;

@@ -1,12 +1,384 @@
-// FP32-compensated step kernels (placeholder until the FP32c path lands).
+// FP32-compensated step kernels of the LandSlide-HySEA solver on MI355X.
+//
+// HBM layout of the state (one ping-pong buffer): 12 FP32 planes of N cells,
+// values h1,q1x,q1y,h2,q2x,q2y then their compensations; depth H is one FP64
+// plane.  Algorithmic traffic per cell-update: 56 B read (24 values + 24
+// compensations + 8 depth) and 48 B written = 104 B.
+//
+// The fused kernel follows the FP64 one (lhysea_fp64.hip): an even-aligned
+// TX x TY tile with a 1-cell halo staged in LDS, the four edge classes in the
+// reference's order with LDS accumulators, the compensated cell update, and
+// a wavefront-shuffle max of the wave-speed sums for the lagged dt.
 #include "lhysea_internal.hpp"
+#include "lhysea_fp32.hpp"
 
 namespace lhysea {
+namespace f32 {
 
-hipError_t f32c_initial_dt(const Consts&, const float*, Ctl*, hipStream_t) { return hipErrorNotSupported; }
-hipError_t f32c_step_fused(const Consts&, const float*, float*, const double*, const StepArgs&, hipStream_t)
+__device__ __forceinline__ double wave_max(double v)
 {
-    return hipErrorNotSupported;
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+template <int NT>
+__device__ __forceinline__ void block_max_atomic(double v, unsigned long long* slot)
+{
+    __shared__ double red[NT / 64];
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int i = 1; i < NT / 64; i++) m = fmax(m, red[i]);
+        if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__device__ __forceinline__ double step_dt(const Ctl* ctl, const double* dt_seq)
+{
+    return dt_seq ? dt_seq[ctl->step] : ctl->dt_cur;
+}
+
+__device__ __forceinline__ K32 make_k32(const Consts& k, double dt)
+{
+    K32 c;
+    c.eps = (float)k.eps_h;
+    c.eps4 = c.eps * c.eps * c.eps * c.eps;
+    c.g = (float)k.g;
+    c.gp = (float)(k.g * (1.0 - k.r));
+    c.r = (float)k.r;
+    c.mu = (float)k.mu;
+    c.peso = (float)k.peso;
+    c.dtf = (float)dt;
+    c.area = (float)k.area;
+    c.len_x = (float)k.dy;
+    c.len_y = (float)k.dx;
+    return c;
+}
+
+// ---------------------------------------------------------------- initial dt
+__global__ void __launch_bounds__(256) k_dt0(Consts k, const float* __restrict__ W, Ctl* ctl)
+{
+    const int nx = k.nx, ny = k.ny;
+    const size_t N = (size_t)nx * ny;
+    const long cidx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const K32 c = make_k32(k, 0.0);
+    double A = 0.0;
+    if (cidx < (long)N) {
+        const int i = (int)(cidx % nx), j = (int)(cidx / nx);
+        float Wc[6], Wo[6], v[4];
+        for (int f = 0; f < 6; f++) Wc[f] = W[f * N + cidx];
+        for (int s = 0; s < 2; s++) {
+            const int je = j + s;
+            if (je == 0 || je == ny) {
+                const float bd = (float)((je == 0) ? k.borde_sup : k.borde_inf);
+                for (int f = 0; f < 6; f++) Wo[f] = Wc[f];
+                Wo[2] *= bd; Wo[5] *= bd;
+                v[je & 1] = edge_dt0(c, Wc, Wo, 0.0f, (je == 0) ? -1.0f : 1.0f, c.len_y);
+            } else {
+                const long o = (s == 0) ? cidx - nx : cidx + nx;
+                for (int f = 0; f < 6; f++) Wo[f] = W[f * N + o];
+                v[je & 1] = (s == 0) ? edge_dt0(c, Wo, Wc, 0.0f, 1.0f, c.len_y) : edge_dt0(c, Wc, Wo, 0.0f, 1.0f, c.len_y);
+            }
+        }
+        for (int s = 0; s < 2; s++) {
+            const int ie = i + s;
+            if (ie == 0 || ie == nx) {
+                const float bd = (float)((ie == 0) ? k.borde_izq : k.borde_der);
+                for (int f = 0; f < 6; f++) Wo[f] = Wc[f];
+                Wo[1] *= bd; Wo[4] *= bd;
+                v[2 + (ie & 1)] = edge_dt0(c, Wc, Wo, (ie == 0) ? -1.0f : 1.0f, 0.0f, c.len_x);
+            } else {
+                const long o = (s == 0) ? cidx - 1 : cidx + 1;
+                for (int f = 0; f < 6; f++) Wo[f] = W[f * N + o];
+                v[2 + (ie & 1)] = (s == 0) ? edge_dt0(c, Wo, Wc, 1.0f, 0.0f, c.len_x) : edge_dt0(c, Wc, Wo, 1.0f, 0.0f, c.len_x);
+            }
+        }
+        A = (double)(((v[0] + v[1]) + v[2]) + v[3]);
+    }
+    block_max_atomic<256>(A, &ctl->maxA_next);
+}
+
+__global__ void k_dt0_finalize(Consts k, Ctl* ctl)
+{
+    if (threadIdx.x == 0) {
+        double maxA = __longlong_as_double((long long)ctl->maxA_next);
+        ctl->dt_cur = dt_from_maxA(maxA, k.cfl, k.area);
+        ctl->maxA_last = maxA;
+        ctl->maxA_next = 0ull;
+    }
+}
+
+__global__ void k_finalize(Consts k, const float* __restrict__ Wn, const double* __restrict__ H, StepArgs a)
+{
+    Ctl* ctl = a.ctl;
+    const size_t N = (size_t)k.nx * k.ny;
+    const int s = ctl->step;
+    const double dt = step_dt(ctl, a.dt_seq);
+    if (a.gseries && s < a.gcap)
+        for (int g = threadIdx.x; g < a.ng; g += blockDim.x) {
+            const size_t c = (size_t)a.gauges[g];
+            const double h1 = (double)Wn[c] + (double)Wn[6 * N + c];
+            const double h2 = (double)Wn[3 * N + c] + (double)Wn[9 * N + c];
+            a.gseries[(size_t)s * a.ng + g] = h1 + h2 - H[c];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (a.dt_hist && s < a.hcap) a.dt_hist[s] = dt;
+        double maxA = __longlong_as_double((long long)ctl->maxA_next);
+        ctl->t = ctl->t + dt;
+        ctl->dt_cur = dt_from_maxA(maxA, k.cfl, k.area);
+        ctl->maxA_last = maxA;
+        ctl->maxA_next = 0ull;
+        ctl->step = s + 1;
+    }
+}
+
+// ----------------------------------------------------------------- fused step
+// LDS: sH (depth), sh1/sh2 (full-precision h as FP64), sq (q hi values),
+// sA (FP32 accumulators: dh1,dq1x,dq1y,dh2,dq2x,dq2y,A).
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_fused(Consts k, const float* __restrict__ W, float* __restrict__ Wn,
+                                              const double* __restrict__ H, StepArgs a, int tiles_x, int tiles_y)
+{
+    constexpr int CW = TX + 2, RW = TY + 2;
+    __shared__ double sH[RW][CW];
+    __shared__ double sh1[RW][CW];
+    __shared__ double sh2[RW][CW];
+    __shared__ float sq[4][RW][CW];
+    __shared__ float sA[7][TY][CW];
+
+    const int nx = k.nx, ny = k.ny;
+    const size_t N = (size_t)nx * ny;
+    const int nb = tiles_x * tiles_y;
+    int b;
+    {
+        const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+        const int q = nb >> 3, rem = nb & 7;
+        b = xcd * q + (xcd < rem ? xcd : rem) + slot;
+    }
+    const int tx = b % tiles_x, ty = b / tiles_x;
+    const int i0 = tx * TX, j0 = ty * TY;
+    const double dt = step_dt(a.ctl, a.dt_seq);
+    const K32 kc = make_k32(k, dt);
+    const int tid = threadIdx.x;
+
+    for (int e = tid; e < RW * CW; e += NT) {
+        const int r = e / CW, c = e % CW;
+        const int gj = j0 + r - 1, gi = i0 + c - 1;
+        if (gj >= 0 && gj < ny && gi >= 0 && gi < nx) {
+            const size_t g = (size_t)gj * nx + gi;
+            sH[r][c] = H[g];
+            sh1[r][c] = (double)W[g] + (double)W[6 * N + g];
+            sh2[r][c] = (double)W[3 * N + g] + (double)W[9 * N + g];
+            sq[0][r][c] = W[N + g];
+            sq[1][r][c] = W[2 * N + g];
+            sq[2][r][c] = W[4 * N + g];
+            sq[3][r][c] = W[5 * N + g];
+        }
+    }
+    for (int e = tid; e < TY * CW; e += NT) {
+        const int r = e / CW, c = e % CW;
+#pragma unroll
+        for (int f = 0; f < 7; f++) sA[f][r][c] = 0.0f;
+    }
+    __syncthreads();
+
+    auto loadW = [&](int r, int c, float* w) {
+        w[0] = (float)sh1[r][c];
+        w[1] = sq[0][r][c];
+        w[2] = sq[1][r][c];
+        w[3] = (float)sh2[r][c];
+        w[4] = sq[2][r][c];
+        w[5] = sq[3][r][c];
+    };
+
+    // y-edge classes C1 then C2
+#pragma unroll 1
+    for (int par = 0; par < 2; par++) {
+        const int nrow = (par == 0) ? (TY / 2 + 1) : (TY / 2);
+        for (int e = tid; e < nrow * CW; e += NT) {
+            const int c = e % CW, jl = par + 2 * (e / CW);
+            const int gi = i0 + c - 1, je = j0 + jl;
+            if (gi < 0 || gi >= nx || je > ny) continue;
+            float W0[6], W1[6];
+            double h10, h20, h11, h21, H0, H1;
+            float a1h1 = 0.0f, a1h2 = 0.0f, n1y;
+            int r0;
+            bool interior;
+            if (je == 0 || je == ny) {
+                r0 = (je == 0) ? jl : jl - 1;
+                const float bd = (float)((je == 0) ? k.borde_sup : k.borde_inf);
+                loadW(r0 + 1, c, W0);
+#pragma unroll
+                for (int f = 0; f < 6; f++) W1[f] = W0[f];
+                W1[2] *= bd; W1[5] *= bd;
+                h10 = h11 = sh1[r0 + 1][c];
+                h20 = h21 = sh2[r0 + 1][c];
+                H0 = H1 = sH[r0 + 1][c];
+                n1y = (je == 0) ? -1.0f : 1.0f;
+                interior = false;
+            } else {
+                r0 = jl - 1;
+                loadW(r0 + 1, c, W0);
+                loadW(r0 + 2, c, W1);
+                h10 = sh1[r0 + 1][c]; h20 = sh2[r0 + 1][c];
+                h11 = sh1[r0 + 2][c]; h21 = sh2[r0 + 2][c];
+                H0 = sH[r0 + 1][c];
+                H1 = sH[r0 + 2][c];
+                n1y = 1.0f;
+                interior = true;
+            }
+            const bool has0 = (r0 >= 0 && r0 < TY), has1 = interior && (r0 + 1 < TY);
+            const float a0h1 = has0 ? sA[0][r0][c] : 0.0f, a0h2 = has0 ? sA[3][r0][c] : 0.0f;
+            if (has1) { a1h1 = sA[0][r0 + 1][c]; a1h2 = sA[3][r0 + 1][c]; }
+            EdgeFlux F;
+            if (edge(kc, W0, W1, h10, h20, h11, h21, H0, H1, 0.0f, n1y, kc.len_y, a0h1, a0h2, a1h1, a1h2,
+                     interior, F)) {
+                if (has0) {
+#pragma unroll
+                    for (int f = 0; f < 6; f++) sA[f][r0][c] -= F.Fm[f];
+                    sA[6][r0][c] += F.lam;
+                }
+                if (has1) {
+#pragma unroll
+                    for (int f = 0; f < 6; f++) sA[f][r0 + 1][c] -= F.Fp[f];
+                    sA[6][r0 + 1][c] += F.lam;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // x-edge classes C3 then C4
+#pragma unroll 1
+    for (int par = 0; par < 2; par++) {
+        const int ncol = (par == 0) ? (TX / 2 + 1) : (TX / 2);
+        for (int e = tid; e < ncol * TY; e += NT) {
+            const int r = e / ncol, il = par + 2 * (e % ncol);
+            const int gj = j0 + r, ie = i0 + il;
+            if (gj >= ny || ie > nx) continue;
+            float W0[6], W1[6];
+            double h10, h20, h11, h21, H0, H1;
+            float a1h1 = 0.0f, a1h2 = 0.0f, n1x;
+            int c0;
+            bool interior;
+            if (ie == 0 || ie == nx) {
+                c0 = (ie == 0) ? il + 1 : il;
+                const float bd = (float)((ie == 0) ? k.borde_izq : k.borde_der);
+                loadW(r + 1, c0, W0);
+#pragma unroll
+                for (int f = 0; f < 6; f++) W1[f] = W0[f];
+                W1[1] *= bd; W1[4] *= bd;
+                h10 = h11 = sh1[r + 1][c0];
+                h20 = h21 = sh2[r + 1][c0];
+                H0 = H1 = sH[r + 1][c0];
+                n1x = (ie == 0) ? -1.0f : 1.0f;
+                interior = false;
+            } else {
+                c0 = il;
+                loadW(r + 1, c0, W0);
+                loadW(r + 1, c0 + 1, W1);
+                h10 = sh1[r + 1][c0]; h20 = sh2[r + 1][c0];
+                h11 = sh1[r + 1][c0 + 1]; h21 = sh2[r + 1][c0 + 1];
+                H0 = sH[r + 1][c0];
+                H1 = sH[r + 1][c0 + 1];
+                n1x = 1.0f;
+                interior = true;
+            }
+            const float a0h1 = sA[0][r][c0], a0h2 = sA[3][r][c0];
+            if (interior) { a1h1 = sA[0][r][c0 + 1]; a1h2 = sA[3][r][c0 + 1]; }
+            EdgeFlux F;
+            if (edge(kc, W0, W1, h10, h20, h11, h21, H0, H1, n1x, 0.0f, kc.len_x, a0h1, a0h2, a1h1, a1h2,
+                     interior, F)) {
+#pragma unroll
+                for (int f = 0; f < 6; f++) sA[f][r][c0] -= F.Fm[f];
+                sA[6][r][c0] += F.lam;
+                if (interior) {
+#pragma unroll
+                    for (int f = 0; f < 6; f++) sA[f][r][c0 + 1] -= F.Fp[f];
+                    sA[6][r][c0 + 1] += F.lam;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // compensated cell update of the tile's own cells
+    KCell kk;
+    kk.eps = kc.eps;
+    kk.eps4 = kc.eps4;
+    kk.r = kc.r;
+    kk.g = kc.g;
+    kk.mu = kc.mu;
+    kk.mfc = (float)k.mfc;
+    kk.mf0 = (float)k.mf0;
+    kk.mfs = (float)k.mfs;
+    kk.vmax1 = (float)k.vmax1;
+    kk.vmax2 = (float)k.vmax2;
+    kk.dtf = kc.dtf;
+    kk.valf = (float)(dt / k.area);
+    double Amax = 0.0;
+    const double tnow = a.ctl->t;
+    for (int e = tid; e < TX * TY; e += NT) {
+        const int r = e / TX, c = e % TX;
+        const int gj = j0 + r, gi = i0 + c;
+        if (gj >= ny || gi >= nx) continue;
+        const size_t g = (size_t)gj * nx + gi;
+        Pair o[6];
+        const double h1d = sh1[r + 1][c + 1], h2d = sh2[r + 1][c + 1];
+        o[0].hi = (float)h1d; o[0].lo = (float)(h1d - (double)o[0].hi);
+        o[3].hi = (float)h2d; o[3].lo = (float)(h2d - (double)o[3].hi);
+        o[1] = Pair{sq[0][r + 1][c + 1], W[7 * N + g]};
+        o[2] = Pair{sq[1][r + 1][c + 1], W[8 * N + g]};
+        o[4] = Pair{sq[2][r + 1][c + 1], W[10 * N + g]};
+        o[5] = Pair{sq[3][r + 1][c + 1], W[11 * N + g]};
+        float ac[6];
+#pragma unroll
+        for (int f = 0; f < 6; f++) ac[f] = sA[f][r][c + 1];
+        Amax = fmax(Amax, (double)sA[6][r][c + 1]);
+        if (a.emax) {
+            const double v = h1d - sH[r + 1][c + 1];
+            if (v > a.emax[2 * g]) {
+                a.emax[2 * g] = v;
+                a.emax[2 * g + 1] = tnow;
+            }
+        }
+        cell_update(kk, o, ac);
+#pragma unroll
+        for (int f = 0; f < 6; f++) {
+            Wn[f * N + g] = o[f].hi;
+            Wn[(6 + f) * N + g] = o[f].lo;
+        }
+    }
+    block_max_atomic<NT>(Amax, &a.ctl->maxA_next);
+}
+
+constexpr int FTX = 62, FTY = 14, FNT = 512;
+
+}  // namespace f32
+
+hipError_t f32c_initial_dt(const Consts& k, const float* W, Ctl* ctl, hipStream_t s)
+{
+    const size_t N = (size_t)k.nx * k.ny;
+    hipLaunchKernelGGL(f32::k_dt0, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, k, W, ctl);
+    hipLaunchKernelGGL(f32::k_dt0_finalize, dim3(1), dim3(64), 0, s, k, ctl);
+    return hipGetLastError();
+}
+
+hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const double* H, const StepArgs& a,
+                           hipStream_t s)
+{
+    const int tiles_x = (k.nx + f32::FTX - 1) / f32::FTX;
+    const int tiles_y = (k.ny + f32::FTY - 1) / f32::FTY;
+    if (a.ev0) hipEventRecord(a.ev0, s);
+    hipLaunchKernelGGL((f32::k_fused<f32::FTX, f32::FTY, f32::FNT>), dim3((unsigned)(tiles_x * tiles_y)),
+                       dim3(f32::FNT), 0, s, k, W, Wn, H, a, tiles_x, tiles_y);
+    if (a.ev1) hipEventRecord(a.ev1, s);
+    hipLaunchKernelGGL(f32::k_finalize, dim3(1), dim3(64), 0, s, k, Wn, H, a);
+    return hipGetLastError();
 }
 
 }  // namespace lhysea

@@ -456,8 +456,10 @@ hipError_t f64_step_fused(const Consts& k, const double* W, double* Wn, const do
 {
     const int tiles_x = (k.nx + f64::FTX - 1) / f64::FTX;
     const int tiles_y = (k.ny + f64::FTY - 1) / f64::FTY;
+    if (a.ev0) hipEventRecord(a.ev0, s);
     hipLaunchKernelGGL((f64::k_fused<f64::FTX, f64::FTY, f64::FNT>), dim3((unsigned)(tiles_x * tiles_y)),
                        dim3(f64::FNT), 0, s, k, W, Wn, H, a, tiles_x, tiles_y);
+    if (a.ev1) hipEventRecord(a.ev1, s);
     hipLaunchKernelGGL(f64::k_finalize, dim3(1), dim3(64), 0, s, k, Wn, H, a);
     return hipGetLastError();
 }

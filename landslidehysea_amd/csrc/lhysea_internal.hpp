@@ -21,6 +21,7 @@ struct StepArgs {
     int gcap;
     double* dt_hist;        // [hcap]
     int hcap;
+    hipEvent_t ev0, ev1;    // when set: recorded around the main step kernel only
 };
 
 // Device buffers of one simulation in FP64.

@@ -123,9 +123,11 @@ SIGNATURES = {
 }
 
 
-def load_library(path: str = LIB_PATH):
+def load_library(path: str | None = None):
     """Load liblhysea_hip.so (raises NativeLibraryMissing when it is not built)."""
     global _lib
+    if path is None:
+        path = LIB_PATH
     if _lib is not None and path == LIB_PATH:
         return _lib
     if not os.path.exists(path):

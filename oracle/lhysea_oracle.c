@@ -38,7 +38,7 @@
 #define LH_SQRT2 1.41421356237309504880168872420969808
 #define LH_BIG 1e30
 
-/* Flat parameter block shared with the ctypes mirror in tests/oracle_lib.py. */
+/* Flat parameter block mirrored by the ctypes structure in oracle/binding.py. */
 typedef struct {
     int nx, ny;
     int friction_law;     /* 0 = Coulomb, 1 = Pouliquen */
@@ -101,8 +101,10 @@ void lh_flux1d(const double w[4], double eps_h, double F[4])
 
 /* Eigenvalue estimate with the reference's compare-swap network
  * (Arista_kernel.cu:152-237).  flag 0: f = (h1,u1,h2,u2); flag 1: f = (h1,q1,h2,q2). */
-void lh_eig(const double f[4], double r, double g, double eps_h, int flag, double D[4])
+void lh_eig(const double fin[4], double r, double g, double eps_h, int flag, double D[4])
 {
+    /* the reference returns a fresh vector; copy so that D may alias the input */
+    const double f[4] = {fin[0], fin[1], fin[2], fin[3]};
     double gp = g * (1.0 - r);
     double q = (flag == 1) ? (f[1] + f[3]) : (f[0] * f[1] + f[2] * f[3]);
     double h = f[0] + f[2];
