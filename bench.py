@@ -85,7 +85,51 @@ def cpu_baseline(sample_steps=3, nx=1024, ny=1024):
             "kind": "port", "sample": f"FP64 oracle, {nx}x{ny} same construction, {sample_steps} steps"}
 
 
+def copy_probe(mib=256):
+    """Plain device-to-device copy of a known size: the calibration kernel for the
+    FETCH_SIZE / WRITE_SIZE counters (torch's copy launches one copy kernel)."""
+    import torch
+    n = mib * 1024 * 1024 // 4
+    x = torch.ones(n, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4}), flush=True)
+
+
+def summarize(dbs):
+    """Per-kernel summary of rocprofv3 result databases (kernel trace and counters)."""
+    import sqlite3
+    for db in dbs:
+        c = sqlite3.connect(db)
+        print(f"# {db}")
+        try:
+            rows = c.execute("select name, total_calls, total_duration, average, percentage from top_kernels").fetchall()
+            if rows:
+                print("kernel | calls | total_us | avg_us | pct")
+                for name, n, tot, avg, pct in rows:
+                    print(f"{name[:110]} | {n} | {tot:.1f} | {avg:.3f} | {pct:.2f}")
+        except sqlite3.Error:
+            pass
+        try:
+            rows = c.execute("select kernel_name, counter_name, count(*), avg(value) from counters_collection "
+                             "group by kernel_name, counter_name").fetchall()
+            if rows:
+                print("kernel | counter | dispatches | avg_value (KiB)")
+                for name, cn, n, v in rows:
+                    print(f"{name[:110]} | {cn} | {n} | {v:.1f}")
+        except sqlite3.Error:
+            pass
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize":
+        summarize(sys.argv[2:])
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "--copy-probe":
+        copy_probe()
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
