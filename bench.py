@@ -61,6 +61,26 @@ def time_gpu(precision, steps, warmup):
             "mcells": N * steps / (total_ms * 1e-3) / 1e6, "t": t, "steps_done": n}
 
 
+def time_adjoint(precision, steps=4000, snaps=64, n=1024, ngauges=16):
+    """Adjoint workload: 1024x1024, one slide, gauge misfit over `steps` steps,
+    binomial checkpointing with `snaps` HBM snapshots."""
+    sc = S.make_scenario(n, n, seed=1024, n_slides=1)
+    H, _ = S.hmin_shift(sc.H)
+    prm = L.Params(nx=n, ny=n, dx=sc.dx, dy=sc.dy, angle_deg=(10.0,), precision=precision)
+    gauges = S.gauge_cells(sc, ngauges, seed=16)
+    obs = np.zeros((steps, len(gauges)))
+    with L.Simulation(prm) as sim:
+        sim.upload(np.ascontiguousarray(sc.state), np.ascontiguousarray(H))
+        t0 = time.perf_counter()
+        res = sim.adjoint(steps, snaps, gauges, obs)
+        wall = time.perf_counter() - t0
+    fwd, rev = res["forward_ms"], res["reverse_ms"]
+    return {"grid": f"{n}x{n}", "steps": steps, "snapshots": snaps, "forward_ms": round(fwd, 2),
+            "reverse_ms": round(rev, 2), "cost_vs_forward": round((fwd + rev) / fwd, 3),
+            "mcells_fwd_plus_adjoint": round(n * n * steps / ((fwd + rev) * 1e-3) / 1e6, 2),
+            "re_advances": res["advances"], "wall_s": round(wall, 2)}
+
+
 def cpu_baseline(sample_steps=3, nx=1024, ny=1024):
     """FP64 oracle (OpenMP) on the host: a bounded sample of the same construction."""
     from oracle import binding as ob
@@ -137,6 +157,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["both", "fp32c", "fp64"], default="both",
                     help="paths to time (profiling runs time one)")
+    ap.add_argument("--adjoint-steps", type=int, default=4000,
+                    help="steps of the adjoint workload (0 skips it)")
+    ap.add_argument("--adjoint-snapshots", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the whole grid lives in one MI355X's HBM); "
@@ -175,6 +198,13 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel_ms": round(head["kernel_ms"], 5), "bytes_per_cell": BYTES_PER_CELL},
     }
+    if args.adjoint_steps > 0:
+        adj = {}
+        if fp32c:
+            adj["fp32c"] = time_adjoint(L.FP32C, args.adjoint_steps, args.adjoint_snapshots)
+        if fp64:
+            adj["fp64"] = time_adjoint(L.FP64, args.adjoint_steps, args.adjoint_snapshots)
+        out["adjoint"] = adj
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

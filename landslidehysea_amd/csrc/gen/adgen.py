@@ -140,8 +140,10 @@ def _ty(dt):
     return "F" if dt == "F" else "double"
 
 
-def emit(prog: Prog, outputs_bar: bool = True) -> str:
-    """C++ device function: void NAME(params..., inputs..., out values..., out-adjoints in, input-adjoints out)."""
+def emit(prog: Prog, forward_only: bool = False) -> str:
+    """C++ device function: void NAME(params..., inputs..., out values..., out-adjoints in, input-adjoints out).
+
+    forward_only: NAME_fwd(params..., inputs..., out values...) with the same arithmetic."""
     L = []
     nodes = prog.nodes
     # which active nodes can reach an output (prune the reverse sweep)
@@ -160,12 +162,14 @@ def emit(prog: Prog, outputs_bar: bool = True) -> str:
         args.append(f"{_ty(n.dt)} {n.code}")
     for name, o in prog.outputs:
         args.append(f"{_ty(o.dt)}& {name}")
-    for name, o in prog.outputs:
-        args.append(f"{_ty(o.dt)} g_{name}")
-    for n in prog.inputs:
-        args.append(f"{_ty(n.dt)}& g_{n.code}")
+    if not forward_only:
+        for name, o in prog.outputs:
+            args.append(f"{_ty(o.dt)} g_{name}")
+        for n in prog.inputs:
+            args.append(f"{_ty(n.dt)}& g_{n.code}")
     L.append("template <typename F>")
-    L.append(f"__device__ __forceinline__ void {prog.name}(" + ", ".join(args) + ")\n{")
+    fname = prog.name + ("_fwd" if forward_only else "")
+    L.append(f"__device__ __forceinline__ void {fname}(" + ", ".join(args) + ")\n{")
     # forward
     for x in prog.seq:
         if isinstance(x, Cond):
@@ -199,6 +203,9 @@ def emit(prog: Prog, outputs_bar: bool = True) -> str:
         L.append(f"    const {t} {n.ref()} = {e};")
     for name, o in prog.outputs:
         L.append(f"    {name} = {o.ref()};")
+    if forward_only:
+        L.append("}")
+        return "\n".join(L) + "\n"
     # reverse
     L.append("    // reverse sweep")
     for n in nodes:

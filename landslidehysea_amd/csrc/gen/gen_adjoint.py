@@ -276,6 +276,7 @@ def main():
     text = ["// Generated by landslidehysea_amd/csrc/gen/gen_adjoint.py -- do not edit.",
             "// Forward recomputation + reverse sweep of the edge and the cell update",
             "// (branches frozen at the forward choices).", ""]
+    text.append(emit(build_edge(), forward_only=True))
     text.append(emit(build_edge()))
     text.append(emit(build_cell()))
     with open(OUT, "w") as f:

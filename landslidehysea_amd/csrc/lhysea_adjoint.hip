@@ -1,0 +1,459 @@
+// Discrete adjoint of the LandSlide-HySEA step on MI355X, with binomial
+// checkpointing of state snapshots in HBM.
+//
+// One reverse step n (state W^n in the working buffer, frozen dt^n, adjoint
+// lambda^{n+1} of the step's output state):
+//   1. recompute the four edge classes (even y, odd y, even x, odd x) with the
+//      generated forward edge, keeping the partial h-sums each class reads
+//      (the positivity limiter's inputs) and the final flux sums;
+//   2. cell adjoint: lambda^{n+1} -> adjoint of the flux sums, the direct part
+//      of lambda^n, and d/dmu (Coulomb coefficient) of the cell update;
+//   3. edge adjoints in reverse class order: each edge adds its state
+//      adjoints to lambda^n and its partial-sum adjoints to the flux-sum
+//      adjoints of its cells, which the earlier classes then see.
+// Within one class no two edges share a cell, so every update is a plain
+// read-modify-write.  The loss is the L2 misfit of eta = h1 + h2 - H at the
+// gauges after every step; its gradient is injected into lambda^n.
+#include <cmath>
+#include <vector>
+#include "lhysea_internal.hpp"
+#include "lhysea_adjoint.hpp"
+
+namespace lhysea {
+namespace adj {
+
+template <typename F>
+struct SV;
+template <>
+struct SV<float> {
+    const float* W;
+    size_t N;
+    __device__ double h1(size_t c) const { return (double)W[c] + (double)W[6 * N + c]; }
+    __device__ double h2(size_t c) const { return (double)W[3 * N + c] + (double)W[9 * N + c]; }
+    __device__ float q(int k, size_t c) const { return W[(k < 2 ? 1 + k : 2 + k) * N + c]; }
+};
+template <>
+struct SV<double> {
+    const double* W;
+    size_t N;
+    __device__ double h1(size_t c) const { return W[c]; }
+    __device__ double h2(size_t c) const { return W[3 * N + c]; }
+    __device__ double q(int k, size_t c) const { return W[(k < 2 ? 1 + k : 2 + k) * N + c]; }
+};
+
+template <typename F>
+struct Work {
+    F* P;          // [3][2][N] partial h1/h2 sums read by classes 1..3
+    F* ACC;        // [6][N] flux sums
+    F* AB;         // [6][N] adjoints of the flux sums
+    double* LAM;   // [6][N] state adjoint (h1,q1x,q1y,h2,q2x,q2y)
+    double* gmu;   // scalar d/dmu
+};
+
+// Edge geometry of edge e in class cls (the reference's class layout,
+// src/GPU/Arista_kernel.cu:651-949).
+struct EdgeRef {
+    size_t c0, c1;
+    bool interior, xedge;
+    float n1x, n1y, len, bd;
+};
+
+__device__ __forceinline__ bool edge_ref(const Consts& k, long e, int cls, EdgeRef& r)
+{
+    const int nx = k.nx, ny = k.ny;
+    if (cls < 2) {
+        const int nrow = (ny - cls) / 2 + 1;
+        if (e >= (long)nrow * nx) return false;
+        const int je = cls + 2 * (int)(e / nx), i = (int)(e % nx);
+        r.xedge = false;
+        r.len = (float)k.dx;
+        r.n1x = 0.0f;
+        if (je == 0 || je == ny) {
+            r.c0 = (je == 0) ? (size_t)i : (size_t)(ny - 1) * nx + i;
+            r.c1 = r.c0;
+            r.interior = false;
+            r.n1y = (je == 0) ? -1.0f : 1.0f;
+            r.bd = (float)((je == 0) ? k.borde_sup : k.borde_inf);
+        } else {
+            r.c0 = (size_t)(je - 1) * nx + i;
+            r.c1 = r.c0 + nx;
+            r.interior = true;
+            r.n1y = 1.0f;
+            r.bd = 1.0f;
+        }
+    } else {
+        const int par = cls - 2, ncol = (nx - par) / 2 + 1;
+        if (e >= (long)ncol * ny) return false;
+        const int j = (int)(e / ncol), ie = par + 2 * (int)(e % ncol);
+        r.xedge = true;
+        r.len = (float)k.dy;
+        r.n1y = 0.0f;
+        if (ie == 0 || ie == nx) {
+            r.c0 = (ie == 0) ? (size_t)j * nx : (size_t)j * nx + nx - 1;
+            r.c1 = r.c0;
+            r.interior = false;
+            r.n1x = (ie == 0) ? -1.0f : 1.0f;
+            r.bd = (float)((ie == 0) ? k.borde_izq : k.borde_der);
+        } else {
+            r.c0 = (size_t)j * nx + ie - 1;
+            r.c1 = r.c0 + 1;
+            r.interior = true;
+            r.n1x = 1.0f;
+            r.bd = 1.0f;
+        }
+    }
+    return true;
+}
+
+// Inputs of one edge: cell 0, and cell 1 or the ghost of cell 0 (normal
+// discharge times borde).
+template <typename F>
+struct EdgeIn {
+    double h10, h20, h11, h21, H0, H1;
+    F q0[4], q1[4];
+};
+
+template <typename F>
+__device__ __forceinline__ void edge_inputs(const SV<F>& sv, const double* H, const EdgeRef& r, EdgeIn<F>& in)
+{
+    const size_t c1 = r.interior ? r.c1 : r.c0;
+    in.h10 = sv.h1(r.c0);
+    in.h20 = sv.h2(r.c0);
+    in.h11 = sv.h1(c1);
+    in.h21 = sv.h2(c1);
+    in.H0 = H[r.c0];
+    in.H1 = H[c1];
+    for (int k = 0; k < 4; k++) {
+        in.q0[k] = sv.q(k, r.c0);
+        in.q1[k] = sv.q(k, c1);
+    }
+    if (!r.interior) {
+        const int nk = r.xedge ? 0 : 1;   // normal component within (qx, qy)
+        in.q1[nk] *= (F)r.bd;
+        in.q1[2 + nk] *= (F)r.bd;
+    }
+}
+
+struct EC {
+    float eps, eps4, g, gp, r, mu, dt, area;
+};
+
+__device__ __forceinline__ EC edge_consts(const Consts& k, double dt)
+{
+    EC c;
+    c.eps = (float)k.eps_h;
+    c.eps4 = c.eps * c.eps * c.eps * c.eps;
+    c.g = (float)k.g;
+    c.gp = (float)(k.g * (1.0 - k.r));
+    c.r = (float)k.r;
+    c.mu = (float)k.mu;
+    c.dt = (float)dt;
+    c.area = (float)k.area;
+    return c;
+}
+
+// 1. forward recomputation of one class, keeping the partial sums it reads
+template <typename F>
+__global__ void __launch_bounds__(256) k_recompute(Consts k, SV<F> sv, const double* H, Work<F> w,
+                                                   const double* dt_hist, int step, int cls)
+{
+    const size_t N = sv.N;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    EdgeRef r;
+    if (!edge_ref(k, e, cls, r)) return;
+    const EC c = edge_consts(k, dt_hist[step]);
+    const F a0h1 = w.ACC[r.c0], a0h2 = w.ACC[3 * N + r.c0];
+    const F a1h1 = r.interior ? w.ACC[r.c1] : F(0), a1h2 = r.interior ? w.ACC[3 * N + r.c1] : F(0);
+    if (cls > 0) {
+        F* P = w.P + (size_t)(cls - 1) * 2 * N;
+        P[r.c0] = a0h1;
+        P[N + r.c0] = a0h2;
+        if (r.interior) {
+            P[r.c1] = a1h1;
+            P[N + r.c1] = a1h2;
+        }
+    }
+    EdgeIn<F> in;
+    edge_inputs(sv, H, r, in);
+    F fm[6], fp[6];
+    adj_edge_fwd<F>((F)c.eps, (F)c.eps4, (F)c.g, (F)c.gp, (F)c.r, (F)c.mu, (F)c.dt, (F)c.area, (F)r.len, (F)r.n1x,
+                    (F)r.n1y, r.interior ? F(1) : F(0), in.H0, in.H1, in.h10, in.h20, in.h11, in.h21, in.q0[0],
+                    in.q0[1], in.q0[2], in.q0[3], in.q1[0], in.q1[1], in.q1[2], in.q1[3], a0h1, a0h2, a1h1, a1h2,
+                    fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fp[0], fp[1], fp[2], fp[3], fp[4], fp[5]);
+    for (int f = 0; f < 6; f++) w.ACC[f * N + r.c0] -= fm[f];
+    if (r.interior)
+        for (int f = 0; f < 6; f++) w.ACC[f * N + r.c1] -= fp[f];
+}
+
+__device__ __forceinline__ double block_sum256(double v)
+{
+    __shared__ double red[4];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) s = red[0] + red[1] + red[2] + red[3];
+    return s;
+}
+
+// 2. cell adjoint: lambda^{n+1} -> flux-sum adjoints, direct lambda^n, d/dmu
+template <typename F>
+__global__ void __launch_bounds__(256) k_cell_adj(Consts k, SV<F> sv, Work<F> w, const double* dt_hist, int step)
+{
+    const size_t N = sv.N;
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double gm = 0.0;
+    if (c < N) {
+        const double dt = dt_hist[step];
+        const F eps = (F)k.eps_h;
+        F o[6] = {(F)sv.h1(c), sv.q(0, c), sv.q(1, c), (F)sv.h2(c), sv.q(2, c), sv.q(3, c)};
+        F a[6], gn[6], n[6], go[6], ga[6], gmu;
+        for (int f = 0; f < 6; f++) {
+            a[f] = w.ACC[f * N + c];
+            gn[f] = (F)w.LAM[f * N + c];
+        }
+        adj_cell<F>(eps, eps * eps * eps * eps, (F)k.r, (F)k.g, (F)k.mfc, (F)k.mf0, (F)k.mfs, (F)dt,
+                    (F)(dt / k.area), (F)k.vmax1, (F)k.vmax2, o[0], o[1], o[2], o[3], o[4], o[5], a[0], a[1], a[2],
+                    a[3], a[4], a[5], (F)k.mu, n[0], n[1], n[2], n[3], n[4], n[5], gn[0], gn[1], gn[2], gn[3], gn[4],
+                    gn[5], go[0], go[1], go[2], go[3], go[4], go[5], ga[0], ga[1], ga[2], ga[3], ga[4], ga[5], gmu);
+        for (int f = 0; f < 6; f++) {
+            w.LAM[f * N + c] = (double)go[f];
+            w.AB[f * N + c] = ga[f];
+        }
+        gm = (double)gmu;
+    }
+    const double s = block_sum256(gm);
+    if (threadIdx.x == 0 && s != 0.0) atomicAdd(w.gmu, s);
+}
+
+// 3. edge adjoints of one class
+template <typename F>
+__global__ void __launch_bounds__(256) k_edge_adj(Consts k, SV<F> sv, const double* H, Work<F> w,
+                                                  const double* dt_hist, int step, int cls)
+{
+    const size_t N = sv.N;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    EdgeRef r;
+    if (!edge_ref(k, e, cls, r)) return;
+    const EC c = edge_consts(k, dt_hist[step]);
+    F a0h1 = F(0), a0h2 = F(0), a1h1 = F(0), a1h2 = F(0);
+    if (cls > 0) {
+        const F* P = w.P + (size_t)(cls - 1) * 2 * N;
+        a0h1 = P[r.c0];
+        a0h2 = P[N + r.c0];
+        if (r.interior) {
+            a1h1 = P[r.c1];
+            a1h2 = P[N + r.c1];
+        }
+    }
+    EdgeIn<F> in;
+    edge_inputs(sv, H, r, in);
+    F gfm[6], gfp[6];
+    for (int f = 0; f < 6; f++) {
+        gfm[f] = -w.AB[f * N + r.c0];
+        gfp[f] = r.interior ? -w.AB[f * N + r.c1] : F(0);
+    }
+    F fm[6], fp[6], gq0[4], gq1[4], ga0h1, ga0h2, ga1h1, ga1h2;
+    double gh10, gh20, gh11, gh21;
+    adj_edge<F>((F)c.eps, (F)c.eps4, (F)c.g, (F)c.gp, (F)c.r, (F)c.mu, (F)c.dt, (F)c.area, (F)r.len, (F)r.n1x,
+                (F)r.n1y, r.interior ? F(1) : F(0), in.H0, in.H1, in.h10, in.h20, in.h11, in.h21, in.q0[0], in.q0[1],
+                in.q0[2], in.q0[3], in.q1[0], in.q1[1], in.q1[2], in.q1[3], a0h1, a0h2, a1h1, a1h2, fm[0], fm[1],
+                fm[2], fm[3], fm[4], fm[5], fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], gfm[0], gfm[1], gfm[2],
+                gfm[3], gfm[4], gfm[5], gfp[0], gfp[1], gfp[2], gfp[3], gfp[4], gfp[5], gh10, gh20, gh11, gh21,
+                gq0[0], gq0[1], gq0[2], gq0[3], gq1[0], gq1[1], gq1[2], gq1[3], ga0h1, ga0h2, ga1h1, ga1h2);
+    double* L = w.LAM;
+    if (r.interior) {
+        L[r.c0] += gh10;
+        L[3 * N + r.c0] += gh20;
+        L[N + r.c0] += (double)gq0[0];
+        L[2 * N + r.c0] += (double)gq0[1];
+        L[4 * N + r.c0] += (double)gq0[2];
+        L[5 * N + r.c0] += (double)gq0[3];
+        L[r.c1] += gh11;
+        L[3 * N + r.c1] += gh21;
+        L[N + r.c1] += (double)gq1[0];
+        L[2 * N + r.c1] += (double)gq1[1];
+        L[4 * N + r.c1] += (double)gq1[2];
+        L[5 * N + r.c1] += (double)gq1[3];
+        w.AB[r.c1] += ga1h1;
+        w.AB[3 * N + r.c1] += ga1h2;
+    } else {
+        // the ghost copies cell 0 with its normal discharge times borde
+        const double bd = (double)r.bd;
+        const int nk = r.xedge ? 0 : 1;
+        double gq[4];
+        for (int q = 0; q < 4; q++) gq[q] = (double)gq0[q] + (double)gq1[q] * (((q & 1) == nk) ? bd : 1.0);
+        L[r.c0] += gh10 + gh11;
+        L[3 * N + r.c0] += gh20 + gh21;
+        L[N + r.c0] += gq[0];
+        L[2 * N + r.c0] += gq[1];
+        L[4 * N + r.c0] += gq[2];
+        L[5 * N + r.c0] += gq[3];
+    }
+    w.AB[r.c0] += ga0h1;
+    w.AB[3 * N + r.c0] += ga0h2;
+}
+
+// loss gradient at the gauges: eta = h1 + h2 - H
+__global__ void k_inject(double* LAM, size_t N, const int* gauges, int ng, const double* G)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ng) {
+        const size_t c = (size_t)gauges[g];
+        // gauges on the same cell add up; one thread handles all duplicates of a cell
+        bool first = true;
+        for (int h = 0; h < g; h++)
+            if (gauges[h] == gauges[g]) first = false;
+        if (!first) return;
+        double s = 0.0;
+        for (int h = g; h < ng; h++)
+            if (gauges[h] == gauges[g]) s += G[h];
+        LAM[c] += s;
+        LAM[3 * N + c] += s;
+    }
+}
+
+__global__ void k_set_step(Ctl* ctl, int step)
+{
+    if (threadIdx.x == 0) ctl->step = step;
+}
+
+}  // namespace adj
+
+// ----------------------------------------------------------------- driver
+// Binomial checkpointing: with c snapshots (the range start included) the
+// fewest re-advances for l steps is T(l, c) = t l - C(c+1+t-1, t-1), t minimal
+// with C(c+t, t) >= l (Griewank & Walther, "revolve", ACM TOMS 26, 2000);
+// the first snapshot goes to the split m minimising m + T(l-m, c-1) + T(m, c).
+static double binom(int c, long t)
+{
+    if (t < 0) return 0.0;
+    double v = 1.0;
+    for (long i = 1; i <= t; i++) {
+        v = v * (double)(c + i) / (double)i;
+        if (v > 1e18) return 1e18;
+    }
+    return v;
+}
+
+long revolve_advances(long l, int c)
+{
+    if (l <= 1) return 0;
+    if (c <= 1) return l * (l - 1) / 2;
+    long t = 0;
+    while (binom(c, t) < (double)l) t++;
+    return (long)(t * l - binom(c + 1, t - 1));
+}
+
+long revolve_split(long l, int c)
+{
+    long best = 1, bestv = -1;
+    for (long m = 1; m < l; m++) {
+        long v = m + revolve_advances(l - m, c - 1) + revolve_advances(m, c);
+        if (bestv < 0 || v < bestv) {
+            bestv = v;
+            best = m;
+        }
+    }
+    return best;
+}
+
+// reverse steps end-1 ... start; the state at start is in slot (S - c)
+static hipError_t treeverse(AdjointHooks& h, int S, long start, long end, int c, bool on_chain)
+{
+    hipError_t e;
+    const long l = end - start;
+    if (l <= 0) return hipSuccess;
+    if (l == 1) {
+        if ((e = h.restore(h.ctx, S - c)) != hipSuccess) return e;
+        h.reverses++;
+        return h.reverse(h.ctx, (int)start);
+    }
+    if (c <= 1) {
+        for (long k = end - 1; k >= start; k--) {
+            if ((e = h.restore(h.ctx, S - c)) != hipSuccess) return e;
+            if (k > start) {
+                if ((e = h.advance(h.ctx, (int)start, (int)(k - start))) != hipSuccess) return e;
+                h.advances += k - start;
+            }
+            h.reverses++;
+            if ((e = h.reverse(h.ctx, (int)k)) != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    const long m = revolve_split(l, c);
+    if (!on_chain) {
+        if ((e = h.restore(h.ctx, S - c)) != hipSuccess) return e;
+        if ((e = h.advance(h.ctx, (int)start, (int)m)) != hipSuccess) return e;
+        h.advances += m;
+        if ((e = h.store(h.ctx, S - c + 1)) != hipSuccess) return e;
+    }
+    if ((e = treeverse(h, S, start + m, end, c - 1, on_chain)) != hipSuccess) return e;
+    return treeverse(h, S, start, start + m, c, false);
+}
+
+// positions of the snapshots the first descent stores: (step, slot)
+void revolve_chain(long N, int S, std::vector<std::pair<long, int>>& out)
+{
+    long x = 0;
+    int c = S;
+    while (N - x > 1 && c > 1) {
+        long m = revolve_split(N - x, c);
+        x += m;
+        out.push_back({x, S - c + 1});
+        c--;
+    }
+}
+
+hipError_t run_treeverse(AdjointHooks& h, int S, long N) { return treeverse(h, S, 0, N, S, true); }
+
+// launch wrappers for one reverse step (four-pass structure)
+template <typename F>
+hipError_t reverse_step_t(const Consts& k, const F* W, const double* H, F* P, F* ACC, F* AB, double* LAM, double* gmu,
+                          const double* dt_hist, int step, hipStream_t s)
+{
+    const size_t N = (size_t)k.nx * k.ny;
+    adj::SV<F> sv{W, N};
+    adj::Work<F> w{P, ACC, AB, LAM, gmu};
+    hipMemsetAsync(ACC, 0, 6 * N * sizeof(F), s);
+    long ne[4];
+    for (int cls = 0; cls < 4; cls++)
+        ne[cls] = (cls < 2) ? (long)((k.ny - cls) / 2 + 1) * k.nx : (long)((k.nx - (cls - 2)) / 2 + 1) * k.ny;
+    for (int cls = 0; cls < 4; cls++)
+        hipLaunchKernelGGL(adj::k_recompute<F>, dim3((unsigned)((ne[cls] + 255) / 256)), dim3(256), 0, s, k, sv, H,
+                           w, dt_hist, step, cls);
+    hipLaunchKernelGGL(adj::k_cell_adj<F>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, k, sv, w, dt_hist,
+                       step);
+    for (int cls = 3; cls >= 0; cls--)
+        hipLaunchKernelGGL(adj::k_edge_adj<F>, dim3((unsigned)((ne[cls] + 255) / 256)), dim3(256), 0, s, k, sv, H,
+                           w, dt_hist, step, cls);
+    return hipGetLastError();
+}
+
+hipError_t adj_reverse_step_f32(const Consts& k, const float* W, const double* H, float* P, float* ACC, float* AB,
+                                double* LAM, double* gmu, const double* dt_hist, int step, hipStream_t s)
+{
+    return reverse_step_t<float>(k, W, H, P, ACC, AB, LAM, gmu, dt_hist, step, s);
+}
+
+hipError_t adj_reverse_step_f64(const Consts& k, const double* W, const double* H, double* P, double* ACC, double* AB,
+                                double* LAM, double* gmu, const double* dt_hist, int step, hipStream_t s)
+{
+    return reverse_step_t<double>(k, W, H, P, ACC, AB, LAM, gmu, dt_hist, step, s);
+}
+
+hipError_t adj_inject(double* LAM, size_t N, const int* gauges, int ng, const double* G, hipStream_t s)
+{
+    if (ng <= 0) return hipSuccess;
+    hipLaunchKernelGGL(adj::k_inject, dim3((ng + 63) / 64), dim3(64), 0, s, LAM, N, gauges, ng, G);
+    return hipGetLastError();
+}
+
+hipError_t adj_set_step(Ctl* ctl, int step, hipStream_t s)
+{
+    hipLaunchKernelGGL(adj::k_set_step, dim3(1), dim3(64), 0, s, ctl, step);
+    return hipGetLastError();
+}
+
+}  // namespace lhysea

@@ -458,6 +458,208 @@ int lhysea_time_steps(lhysea_sim* s, int32_t nsteps, double* total_ms, double* k
     return LHYSEA_OK;
 }
 
+namespace {
+
+// Context of one adjoint run: the simulation, the snapshot store and the
+// reverse-step workspace (all in HBM).
+struct AdjCtx {
+    lhysea_sim* s;
+    char* snap;            // S snapshots of the state
+    size_t state_bytes;
+    void* P;               // partial sums [3][2][N]
+    void* ACC;             // [6][N]
+    void* AB;              // [6][N]
+    double* LAM;           // [6][N]
+    double* gmu;
+    int* gauges;
+    double* gser;          // gauge series of the forward pass
+    double* G;             // [nsteps][ng] loss gradient wrt eta
+    int ng;
+};
+
+void* state_ptr(lhysea_sim* s) { return s->p.precision == LHYSEA_FP64 ? (void*)s->b64.W[s->cur] : (void*)s->b32.W[s->cur]; }
+
+hipError_t hook_advance(void* c, int k0, int n)
+{
+    AdjCtx* a = (AdjCtx*)c;
+    hipError_t e = adj_set_step(a->s->ctl, k0, a->s->stream);
+    if (e != hipSuccess) return e;
+    return launch_steps(a->s, n) == LHYSEA_OK ? hipSuccess : hipErrorUnknown;
+}
+
+hipError_t hook_store(void* c, int slot)
+{
+    AdjCtx* a = (AdjCtx*)c;
+    return hipMemcpyAsync(a->snap + (size_t)slot * a->state_bytes, state_ptr(a->s), a->state_bytes,
+                          hipMemcpyDeviceToDevice, a->s->stream);
+}
+
+hipError_t hook_restore(void* c, int slot)
+{
+    AdjCtx* a = (AdjCtx*)c;
+    return hipMemcpyAsync(state_ptr(a->s), a->snap + (size_t)slot * a->state_bytes, a->state_bytes,
+                          hipMemcpyDeviceToDevice, a->s->stream);
+}
+
+hipError_t hook_reverse(void* c, int k)
+{
+    AdjCtx* a = (AdjCtx*)c;
+    lhysea_sim* s = a->s;
+    hipError_t e;
+    if (s->p.precision == LHYSEA_FP64)
+        e = adj_reverse_step_f64(s->k, s->b64.W[s->cur], s->b64.H, (double*)a->P, (double*)a->ACC, (double*)a->AB,
+                                 a->LAM, a->gmu, s->dt_hist, k, s->stream);
+    else
+        e = adj_reverse_step_f32(s->k, s->b32.W[s->cur], s->b32.H, (float*)a->P, (float*)a->ACC, (float*)a->AB,
+                                 a->LAM, a->gmu, s->dt_hist, k, s->stream);
+    if (e != hipSuccess) return e;
+    if (k >= 1) e = adj_inject(a->LAM, s->N, a->gauges, a->ng, a->G + (size_t)(k - 1) * a->ng, s->stream);
+    return e;
+}
+
+}  // namespace
+
+int lhysea_adjoint(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const int32_t* gauges, int32_t ng,
+                   const double* obs, double* loss, double* grad_W0, double* grad_mu, double* grad_angle1,
+                   double* stats)
+{
+    if (!s || nsteps < 1 || nsnap < 1 || ng < 0 || (ng > 0 && (!gauges || !obs)))
+        return fail(LHYSEA_ERR_ARGUMENT, "bad adjoint arguments");
+    if (nsteps > s->hcap) return fail(LHYSEA_ERR_ARGUMENT, "too many steps for the dt history");
+    if (s->p.friction_law != 0) return fail(LHYSEA_ERR_ARGUMENT, "the adjoint implements the Coulomb law");
+    for (int g = 0; g < ng; g++)
+        if (gauges[g] < 0 || (size_t)gauges[g] >= s->N) return fail(LHYSEA_ERR_ARGUMENT, "gauge cell out of range");
+    const size_t N = s->N;
+    const bool f64 = s->p.precision == LHYSEA_FP64;
+    const size_t fsz = f64 ? sizeof(double) : sizeof(float);
+    AdjCtx a{};
+    a.s = s;
+    a.state_bytes = 48 * N;       // 6 doubles or 12 floats per cell
+    a.ng = ng;
+    int rc = LHYSEA_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    const double* saved_seq = s->dt_seq;
+    const bool saved_emax = s->track_emax;
+    auto done = [&](int code) {
+        hipStreamSynchronize(s->stream);
+        hipFree(a.snap);
+        hipFree(a.P);
+        hipFree(a.ACC);
+        hipFree(a.AB);
+        hipFree(a.LAM);
+        hipFree(a.gmu);
+        hipFree(a.G);
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        if (e2) hipEventDestroy(e2);
+        s->dt_seq = const_cast<double*>(saved_seq);
+        s->track_emax = saved_emax;
+        // the gauge buffers belong to this run; the handle's state must be
+        // uploaded again before further steps
+        hipFree(a.gser);
+        if (a.gauges) hipFree(a.gauges);
+        if (s->gauges == a.gauges) s->gauges = nullptr;
+        if (s->gseries == a.gser) s->gseries = nullptr;
+        s->ng = 0;
+        s->gcap = 0;
+        return code;
+    };
+    if (hipMalloc(&a.snap, (size_t)nsnap * a.state_bytes) != hipSuccess || hipMalloc(&a.P, 6 * N * fsz) != hipSuccess ||
+        hipMalloc(&a.ACC, 6 * N * fsz) != hipSuccess || hipMalloc(&a.AB, 6 * N * fsz) != hipSuccess ||
+        hipMalloc(&a.LAM, 6 * N * sizeof(double)) != hipSuccess || hipMalloc(&a.gmu, sizeof(double)) != hipSuccess ||
+        hipMalloc(&a.G, (size_t)nsteps * (ng > 0 ? ng : 1) * sizeof(double)) != hipSuccess)
+        return done(fail(LHYSEA_ERR_GPU_MEMORY, "adjoint workspace allocation failed"));
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventCreate(&e2);
+
+    // 1. forward pass from the uploaded state: dt history, gauge series and the
+    //    snapshots of the first descent of the checkpoint tree
+    rc = lhysea_set_gauges(s, gauges, ng, nsteps);
+    if (rc) return done(rc);
+    a.gauges = s->gauges;
+    a.gser = s->gseries;
+    s->track_emax = false;
+    s->dt_seq = nullptr;
+    Ctl c0;
+    HIPCHK(hipMemcpy(&c0, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    if (c0.step != 0) return done(fail(LHYSEA_ERR_ARGUMENT, "the adjoint starts from a freshly uploaded state"));
+    if (!(c0.dt_cur > 0)) {
+        rc = lhysea_initial_dt(s, nullptr);
+        if (rc) return done(rc);
+    }
+    std::vector<std::pair<long, int>> chain;
+    revolve_chain(nsteps, nsnap, chain);
+    hipEventRecord(e0, s->stream);
+    if (hook_store(&a, 0) != hipSuccess) return done(fail(LHYSEA_ERR_HIP, "snapshot copy failed"));
+    long pos = 0;
+    for (auto& ch : chain) {
+        rc = launch_steps(s, (int)(ch.first - pos));
+        if (rc) return done(rc);
+        if (hook_store(&a, ch.second) != hipSuccess) return done(fail(LHYSEA_ERR_HIP, "snapshot copy failed"));
+        pos = ch.first;
+    }
+    rc = launch_steps(s, (int)(nsteps - pos));
+    if (rc) return done(rc);
+    hipEventRecord(e1, s->stream);
+
+    // 2. loss and its gradient with respect to the gauge series
+    std::vector<double> gs((size_t)nsteps * ng), G((size_t)nsteps * ng);
+    int32_t nrec = 0;
+    rc = lhysea_get_gauges(s, gs.data(), &nrec);
+    if (rc) return done(rc);
+    double L = 0.0;
+    for (size_t i = 0; i < gs.size(); i++) {
+        const double d = gs[i] - obs[i];
+        L += d * d;
+        G[i] = 2.0 * d;
+    }
+    if (loss) *loss = L;
+    if (ng > 0) HIPCHK(hipMemcpy(a.G, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice));
+    // gauges off for the re-advances; the dt history is frozen from here on
+    s->gseries = nullptr;
+    s->ng = 0;
+    s->dt_seq = s->dt_hist;
+    HIPCHK(hipMemsetAsync(a.LAM, 0, 6 * N * sizeof(double), s->stream));
+    HIPCHK(hipMemsetAsync(a.gmu, 0, sizeof(double), s->stream));
+    if (hook_restore(&a, 0) != hipSuccess) return done(fail(LHYSEA_ERR_HIP, "snapshot copy failed"));
+    if (adj_inject(a.LAM, N, a.gauges, ng, a.G + (size_t)(nsteps - 1) * ng, s->stream) != hipSuccess)
+        return done(fail(LHYSEA_ERR_HIP, "loss injection failed"));
+
+    // 3. reverse sweep with binomial checkpointing
+    AdjointHooks h{hook_advance, hook_store, hook_restore, hook_reverse, &a, 0, 0};
+    hipError_t he = run_treeverse(h, nsnap, nsteps);
+    if (he != hipSuccess) return done(hip_fail(he, "adjoint sweep"));
+    hipEventRecord(e2, s->stream);
+    HIPCHK(hipStreamSynchronize(s->stream));
+    he = hipGetLastError();
+    if (he != hipSuccess) return done(hip_fail(he, "adjoint sweep"));
+
+    std::vector<double> lam(6 * N);
+    double gm = 0.0;
+    HIPCHK(hipMemcpy(lam.data(), a.LAM, 6 * N * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&gm, a.gmu, sizeof(double), hipMemcpyDeviceToHost));
+    if (grad_W0) std::copy(lam.begin(), lam.end(), grad_W0);
+    if (grad_mu) *grad_mu = gm;
+    if (grad_angle1) {
+        // mu = |tan a| L/H  ->  dmu/da = sign(tan a) (1 + tan^2 a) L/H
+        const double t = std::tan(s->p.angle1);
+        *grad_angle1 = gm * (t >= 0 ? 1.0 : -1.0) * (1.0 + t * t) * (s->p.L / s->p.H);
+    }
+    if (stats) {
+        float f_ms = 0, r_ms = 0;
+        hipEventElapsedTime(&f_ms, e0, e1);
+        hipEventElapsedTime(&r_ms, e1, e2);
+        stats[0] = f_ms;                       // forward pass (with first-descent snapshots)
+        stats[1] = r_ms;                       // reverse sweep (re-advances + reverse steps)
+        stats[2] = (double)h.advances;         // re-advanced forward steps
+        stats[3] = (double)h.reverses;         // reverse steps
+        stats[4] = (double)nsnap;
+        stats[5] = (double)revolve_advances(nsteps, nsnap);
+    }
+    return done(LHYSEA_OK);
+}
+
 int lhysea_shallow_water(const lhysea_params* p, double* W, const double* H, double t_end, double* eta_max,
                          double* eta_max_time, double* elapsed_s, int32_t* steps_done)
 {

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <string>
+#include <utility>
+#include <vector>
 #include "lhysea_common.hpp"
 
 namespace lhysea {
@@ -49,5 +51,26 @@ hipError_t f64_step_fused(const Consts& k, const double* W, double* Wn, const do
 hipError_t f32c_initial_dt(const Consts& k, const float* W, Ctl* ctl, hipStream_t s);
 hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const double* H, const StepArgs& a,
                            hipStream_t s);
+
+// Adjoint (lhysea_adjoint.hip)
+hipError_t adj_reverse_step_f32(const Consts& k, const float* W, const double* H, float* P, float* ACC, float* AB,
+                                double* LAM, double* gmu, const double* dt_hist, int step, hipStream_t s);
+hipError_t adj_reverse_step_f64(const Consts& k, const double* W, const double* H, double* P, double* ACC,
+                                double* AB, double* LAM, double* gmu, const double* dt_hist, int step, hipStream_t s);
+hipError_t adj_inject(double* LAM, size_t N, const int* gauges, int ng, const double* G, hipStream_t s);
+hipError_t adj_set_step(Ctl* ctl, int step, hipStream_t s);
+
+struct AdjointHooks {
+    hipError_t (*advance)(void* ctx, int k0, int n);
+    hipError_t (*store)(void* ctx, int slot);
+    hipError_t (*restore)(void* ctx, int slot);
+    hipError_t (*reverse)(void* ctx, int k);
+    void* ctx;
+    long advances, reverses;
+};
+long revolve_advances(long l, int c);
+long revolve_split(long l, int c);
+void revolve_chain(long N, int S, std::vector<std::pair<long, int>>& out);
+hipError_t run_treeverse(AdjointHooks& h, int S, long N);
 
 }  // namespace lhysea

@@ -119,6 +119,8 @@ SIGNATURES = {
     "lhysea_dt_history": (C.c_int, [C.c_void_p, _dp, C.c_int32, _ip]),
     "lhysea_set_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
     "lhysea_time_steps": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
+    "lhysea_adjoint": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, C.c_int32, _dp, _dp, _dp, _dp, _dp,
+                                 _dp]),
     "lhysea_shallow_water": (C.c_int, [C.POINTER(CParams), _dp, _dp, C.c_double, _dp, _dp, _dp, _ip]),
 }
 
@@ -249,6 +251,21 @@ class Simulation:
         tot, ker = C.c_double(), C.c_double()
         _check(self.lib.lhysea_time_steps(self.h, nsteps, C.byref(tot), C.byref(ker) if per_kernel else None))
         return tot.value, (ker.value if per_kernel else None)
+
+
+    def adjoint(self, nsteps: int, nsnap: int, gauges, obs):
+        """Gauge-misfit loss over nsteps steps from the uploaded state and its
+        gradients: dL/dW0 (6, ny, nx), dL/dmu, dL/dangle1 (radians)."""
+        g = np.ascontiguousarray(gauges, dtype=np.int32)
+        obs = np.ascontiguousarray(obs, dtype=np.float64).reshape(nsteps, len(g))
+        gW = np.zeros((6, self.params.ny, self.params.nx))
+        loss, gmu, gang = C.c_double(), C.c_double(), C.c_double()
+        stats = np.zeros(6)
+        _check(self.lib.lhysea_adjoint(self.h, nsteps, nsnap, g.ctypes.data_as(_ip), len(g), _dptr(obs),
+                                       C.byref(loss), _dptr(gW), C.byref(gmu), C.byref(gang), _dptr(stats)))
+        return {"loss": loss.value, "grad_W0": gW, "grad_mu": gmu.value, "grad_angle1": gang.value,
+                "forward_ms": stats[0], "reverse_ms": stats[1], "advances": int(stats[2]),
+                "reverse_steps": int(stats[3]), "snapshots": int(stats[4]), "optimal_advances": int(stats[5])}
 
 
 def shallow_water(params: Params, W: np.ndarray, H: np.ndarray, t_end: float):

@@ -1,0 +1,160 @@
+"""Discrete adjoint on the GPU vs central finite differences of the FP64 oracle.
+
+The loss is the gauge misfit L = sum_n sum_g (eta_g^n - obs_g^n)^2 with
+observations from an oracle run of a thicker slide.  Finite differences use
+the oracle with the dt sequence frozen at the unperturbed run (the adjoint
+differentiates with dt and every branch held at the forward choices).
+
+Stated bars: FP64 adjoint within 1e-4 relative of the FD gradient; the FP32
+adjoint of the compensated path within 2e-2 (its forward already differs
+from FP64 by FP32 round-off amplified through the branchy slide dynamics).
+Finite-difference cells are chosen inside the slide (h2 > 5 m): at h = 0 the
+model is not differentiable (sqrt(h) in the Roe average).  The reference's
+scheme is branchy (Coulomb stop, wet/dry, positivity limiter), so the loss has
+kinks and jumps: where the one-sided differences agree the adjoint is
+checked against the central difference; at a branch point it must equal the
+one-sided derivative of one side (the side whose branches the forward took).
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from tests.cases import case, oracle_params
+from landslidehysea_amd import solver as L
+from landslidehysea_amd.autograd import GaugeMisfit, MisfitSpec
+from landslidehysea_amd.scenario import gauge_cells
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def setup(oracle, precision, nx=48, ny=36, N=40):
+    sc, W, H, prm = case(nx, ny, angle=2.0, semi_axes=(10.0, 7.0), precision=precision)
+    gauges = gauge_cells(sc, 6, seed=3)
+    po = oracle_params(oracle, prm)
+    Wp = W.copy()
+    Wp[3] *= 1.1
+    Wp[0] = np.maximum(0.0, (W[0] + W[3]) - Wp[3])
+    obs = oracle.run(po, Wp, H, N, gauges=gauges)["gauges"]
+    r0 = oracle.run(po, W.copy(), H, N, gauges=gauges)
+    return sc, W, H, prm, po, gauges, obs, r0["dt_hist"]
+
+
+def fd_loss(oracle, po, W, H, dts, gauges, obs):
+    g = oracle.run_fixed_dt(po, W.copy(), H, dts, gauges)
+    return float(((g - obs) ** 2).sum())
+
+
+@pytest.mark.parametrize("precision,tol", [(L.FP64, 1e-4), (L.FP32C, 2e-2)])
+def test_adjoint_matches_oracle_fd(oracle, precision, tol):
+    N = 40
+    sc, W, H, prm, po, gauges, obs, dts = setup(oracle, precision, N=N)
+    say(f"adjoint precision={precision} 48x36 N={N} snapshots=8")
+    with L.Simulation(prm) as sim:
+        sim.upload(W, H)
+        res = sim.adjoint(N, 8, gauges, obs)
+    say("loss", res["loss"], "advances", res["advances"], "reverse steps", res["reverse_steps"])
+    assert res["reverse_steps"] == N
+    L0 = fd_loss(oracle, po, W, H, dts, gauges, obs)
+    assert abs(res["loss"] - L0) <= (1e-9 if precision == L.FP64 else 5e-2) * L0
+    rng = np.random.default_rng(5)
+    cells = rng.choice(np.flatnonzero(W[3].ravel() > 5.0), 8, replace=False)
+    nx = prm.nx
+    L0 = fd_loss(oracle, po, W, H, dts, gauges, obs)
+    smooth = 0
+    for c in cells:
+        j, i = divmod(int(c), nx)
+        for k in (0, 3):
+            eps = 1e-5 * max(1.0, W[k, j, i])
+            Wp, Wm = W.copy(), W.copy()
+            Wp[k, j, i] += eps
+            Wm[k, j, i] -= eps
+            lp = fd_loss(oracle, po, Wp, H, dts, gauges, obs)
+            lm = fd_loss(oracle, po, Wm, H, dts, gauges, obs)
+            fp, fm, fc = (lp - L0) / eps, (L0 - lm) / eps, (lp - lm) / (2 * eps)
+            ad = res["grad_W0"][k, j, i]
+            say(f"cell {c} field {k}: adjoint {ad:.6e} central {fc:.6e} one-sided {fp:.6e} {fm:.6e}")
+            if abs(fp - fm) <= 1e-3 * abs(fc) + 1e-12:
+                smooth += 1
+                assert abs(ad - fc) <= tol * abs(fc) + 1e-12
+            else:
+                # a branch (Coulomb stop, wet/dry, limiter) flips on one side of this
+                # state: the frozen-branch adjoint is the derivative of the other side
+                assert min(abs(ad - fp), abs(ad - fm)) <= max(tol, 1e-3) * max(abs(fp), abs(fm)) + 1e-12
+    assert smooth >= 8
+    da = 1e-6
+    pp, pm = oracle_params(oracle, prm), oracle_params(oracle, prm)
+    for p, s in ((pp, 1), (pm, -1)):
+        p.angle1 += s * da
+        p.angle2 += s * da
+        p.angle3 += s * da
+        p.angle4 += s * da
+    fd = (fd_loss(oracle, pp, W, H, dts, gauges, obs) - fd_loss(oracle, pm, W, H, dts, gauges, obs)) / (2 * da)
+    say(f"angle: adjoint {res['grad_angle1']:.6e} fd {fd:.6e}")
+    assert abs(res["grad_angle1"] - fd) <= tol * abs(fd)
+
+
+def test_autograd_wrapper_gradients(oracle):
+    """torch.autograd.Function over the C-ABI pair; gradients checked against oracle FD.
+
+    Coupled mode (water at rest over the slide: h1 = surface - h2) must equal
+    dL/dh2 - dL/dh1 of the uncoupled run where the cell is wet."""
+    N = 30
+    sc, W, H, prm, po, gauges, obs, dts = setup(oracle, L.FP64, N=N)
+    obs = obs[:N]
+    grads = {}
+    for couple in (False, True):
+        spec = MisfitSpec(prm, W, H, gauges, obs, N, 6, couple_water=couple)
+        h2 = torch.tensor(W[3], dtype=torch.float64, requires_grad=True)
+        ang = torch.tensor(math.radians(2.0), dtype=torch.float64, requires_grad=True)
+        loss = GaugeMisfit.apply(h2, ang, spec)
+        loss.backward()
+        grads[couple] = (float(loss), h2.grad.numpy().copy(), float(ang.grad))
+        say("autograd loss", float(loss), "coupled" if couple else "uncoupled")
+    L0 = fd_loss(oracle, po, W, H, dts, gauges, obs)
+    assert abs(grads[False][0] - L0) <= 1e-9 * L0
+    assert grads[False][0] == grads[True][0]
+    rng = np.random.default_rng(9)
+    checked = 0
+    for c in rng.choice(np.flatnonzero(W[3].ravel() > 5.0), 6, replace=False):
+        j, i = divmod(int(c), prm.nx)
+        eps = 1e-5 * W[3, j, i]
+        Wp, Wm = W.copy(), W.copy()
+        Wp[3, j, i] += eps
+        Wm[3, j, i] -= eps
+        lp = fd_loss(oracle, po, Wp, H, dts, gauges, obs)
+        lm = fd_loss(oracle, po, Wm, H, dts, gauges, obs)
+        fp, fm, fc = (lp - L0) / eps, (L0 - lm) / eps, (lp - lm) / (2 * eps)
+        ad = grads[False][1][j, i]
+        say(f"cell {c}: autograd {ad:.6e} central {fc:.6e} one-sided {fp:.6e} {fm:.6e}")
+        if abs(fp - fm) <= 1e-3 * abs(fc) + 1e-12:
+            checked += 1
+            assert abs(ad - fc) <= 1e-4 * abs(fc) + 1e-12
+    assert checked >= 3
+    with L.Simulation(prm) as sim:
+        sim.upload(W, H)
+        res = sim.adjoint(N, 6, gauges, obs)
+    wet = (W[0] + W[3] - W[3]) > 0.0
+    expect = res["grad_W0"][3] - np.where(wet, res["grad_W0"][0], 0.0)
+    np.testing.assert_allclose(grads[True][1], expect, rtol=1e-12, atol=1e-18)
+    assert grads[True][2] == pytest.approx(res["grad_angle1"], rel=1e-12)
+
+
+def test_checkpoint_schedule_is_binomial():
+    """Re-advance counts of the checkpoint tree equal the binomial optimum."""
+    sc, W, H, prm = case(24, 18, angle=2.0, semi_axes=(5.0, 4.0), precision=L.FP32C)
+    gauges = gauge_cells(sc, 3, seed=1)
+    for N, S in ((17, 3), (40, 5), (25, 1)):
+        obs = np.zeros((N, len(gauges)))
+        with L.Simulation(prm) as sim:
+            sim.upload(W, H)
+            res = sim.adjoint(N, S, gauges, obs)
+        chain = res["optimal_advances"] - res["advances"]
+        say(f"N={N} S={S}: re-advances {res['advances']} optimal total {res['optimal_advances']}")
+        assert res["reverse_steps"] == N
+        assert 0 <= chain < N
