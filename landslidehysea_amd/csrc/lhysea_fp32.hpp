@@ -29,18 +29,20 @@ __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float fexp(float x) { return __expf(x); }
 
-// s(h): U(h,q) = q * s(h)
+// s(h): U(h,q) = q * s(h) = sqrt2 h / sqrt(h^4 + max(h,eps)^4)  (= 1/h for h >= eps);
+// one reciprocal square root, no branch
 __device__ __forceinline__ float sdes(float h, float eps, float eps4)
 {
-    float h2 = h * h;
-    return (h >= eps) ? rcp(h) : SQRT2F * h * rsq(h2 * h2 + eps4);
+    float h2 = h * h, m = fmaxf(h, eps), m2 = m * m;
+    return SQRT2F * h * rsq(h2 * h2 + m2 * m2);
 }
 
 // Two-layer eigenvalue estimate + the reference's compare-swap network
 // (src/GPU/Arista_kernel.cu:152-237), FLAG 0: (h1,u1,h2,u2), FLAG 1: (h1,q1,h2,q2).
+// s0, s2: s(f0), s(f2) of the layers (FLAG 1 only); igp = 1/gp
 template <int FLAG>
-__device__ __forceinline__ void eig(float f0, float f1, float f2, float f3, float g, float gp, float eps,
-                                    float eps4, float& Dx, float& Dy, float& Dz, float& Dw)
+__device__ __forceinline__ void eig(float f0, float f1, float f2, float f3, float s0, float s2, float g, float gp,
+                                    float igp, float eps, float eps4, float& Dx, float& Dy, float& Dz, float& Dw)
 {
     float h = f0 + f2;
     float sh = sdes(h, eps, eps4);
@@ -55,12 +57,12 @@ __device__ __forceinline__ void eig(float f0, float f1, float f2, float f3, floa
         u1 = f1;
         u2 = f3;
     } else {
-        u1 = f1 * sdes(f0, eps, eps4);
-        u2 = f3 * sdes(f2, eps, eps4);
+        u1 = f1 * s0;
+        u2 = f3 * s2;
         uu = (u1 * f2 + u2 * f0) * sh;
     }
     float du = u1 - u2;
-    float aux = 1.0f - du * du * sh * rcp(gp);
+    float aux = 1.0f - du * du * sh * igp;
     float ci = fsqrt(gp * f0 * f2 * sh * fabsf(aux));
     Dy = uu - ci;
     Dz = uu + ci;
@@ -82,6 +84,7 @@ struct EdgeFlux {
 // Per-run FP32 constants.
 struct K32 {
     float eps, eps4, g, gp, r, mu, peso, dtf, area, len_x, len_y;
+    float igp, ten_eps, idt_eps, bq;   // 1/gp, 10/eps, 1/(dt+EPS), dt/area
 };
 
 // One edge (src/GPU/Arista_kernel.cu:324-645) with FP32 arithmetic and FP64
@@ -122,16 +125,18 @@ __device__ __forceinline__ bool edge(const K32& k, const float* W0, const float*
 
     float tp1 = g * h1ij * deta1;
     float tp3 = g * h2ij * ((1.0f - r) * deta2 + r * deta1);
-    float x = 10.0f * h1ij * rcp(eps);
-    float fsc = 1.0f - r * (1.0f - fexp(-(x * x)));
+    float x = h1ij * k.ten_eps;
+    float ex = 0.0f;
+    if (x < 9.5f) ex = fexp(-(x * x));         // exp(-x^2) is 0 in FP32 beyond x = 9.4
+    float fsc = 1.0f - r * (1.0f - ex);
     float sc = fsc * k.mu * g * h2ij;
     bool coul = fabsf(h2ij * u2ij) < k.peso * sc * k.dtf;
     float t23 = coul ? (g * h2ij * r * deta1) : tp3;
 
     float Dx, Dy, Dz, Dw, Lx, Ly, Lz, Lw, Rx, Ry, Rz, Rw;
-    eig<0>(h1ij, u1ij, h2ij, u2ij, g, k.gp, eps, eps4, Dx, Dy, Dz, Dw);
-    eig<1>(w00, w01, w02, w03, g, k.gp, eps, eps4, Lx, Ly, Lz, Lw);
-    eig<1>(w10, w11, w12, w13, g, k.gp, eps, eps4, Rx, Ry, Rz, Rw);
+    eig<0>(h1ij, u1ij, h2ij, u2ij, 0.0f, 0.0f, g, k.gp, k.igp, eps, eps4, Dx, Dy, Dz, Dw);
+    eig<1>(w00, w01, w02, w03, s00, s02, g, k.gp, k.igp, eps, eps4, Lx, Ly, Lz, Lw);
+    eig<1>(w10, w11, w12, w13, s10, s12, g, k.gp, k.igp, eps, eps4, Rx, Ry, Rz, Rw);
     float aut1 = fminf(Dx, fminf(Lx, Rx));
     float aut3 = fmaxf(Dw, fmaxf(Lw, Rw));
     float maxl = fmaxf(fmaxf(fabsf(aut3), fabsf(aut1)), fmaxf(fabsf(u1ij), fabsf(u2ij)));
@@ -144,9 +149,11 @@ __device__ __forceinline__ bool edge(const K32& k, const float* W0, const float*
     float d13 = aut1 - aut3, d12 = aut1 - aut2, d23 = aut2 - aut3;
     if ((fabsf(d13) >= LH_EPSF) && (fabsf(d12) >= LH_EPSF) && (fabsf(d23) >= LH_EPSF)) {
         if ((h1ij >= eps) && (h2ij >= eps)) {
-            float A = fabsf(aut1) * rcp(d13 * d12);
-            float B = -fabsf(aut2) * rcp(d12 * d23);
-            float C = fabsf(aut3) * rcp(d13 * d23);
+            // A = |a1|/(d13 d12), B = -|a2|/(d12 d23), C = |a3|/(d13 d23) over one reciprocal
+            float iD = rcp(d13 * d12 * d23);
+            float A = fabsf(aut1) * d23 * iD;
+            float B = -fabsf(aut2) * d13 * iD;
+            float C = fabsf(aut3) * d12 * iD;
             c0 = A * aut2 * aut3 + B * aut1 * aut3 + C * aut1 * aut2;
             c1 = -A * (aut2 + aut3) - B * (aut1 + aut3) - C * (aut1 + aut2);
             c2 = A + B + C;
@@ -208,20 +215,22 @@ __device__ __forceinline__ bool edge(const K32& k, const float* W0, const float*
     P[4] = len * (Fp3 * n1x + T2 * n1y); P[5] = len * (Fp3 * n1y - T2 * n1x);
 
     // positivity with the partial accumulators (src/GPU/Arista_kernel.cu:554-609)
-    const float area = k.area, dt = k.dtf;
-    const float bq = dt * rcp(area);
+    // P[h] = -Mn[h]: at most one of the two outflow candidates exists per layer
+    const float area = k.area, dt = k.dtf, bq = k.bq;
     float dt0 = 1e30f, dt1 = 1e30f;
-    if (Mn[0] > 0.0f) dt0 = (W0[0] + bq * a0h1) * area * rcp(Mn[0] + LH_EPSF);
-    if (interior && P[0] > 0.0f) dt0 = fminf(dt0, (W1[0] + bq * a1h1) * area * rcp(P[0] + LH_EPSF));
-    if (Mn[3] > 0.0f) dt1 = (W0[3] + bq * a0h2) * area * rcp(Mn[3] + LH_EPSF);
-    if (interior && P[3] > 0.0f) dt1 = fminf(dt1, (W1[3] + bq * a1h2) * area * rcp(P[3] + LH_EPSF));
+    {
+        const bool out0 = Mn[0] > 0.0f, in0 = interior && (Mn[0] < 0.0f);
+        if (out0 || in0) dt0 = (out0 ? (W0[0] + bq * a0h1) : (W1[0] + bq * a1h1)) * area * rcp(fabsf(Mn[0]) + LH_EPSF);
+        const bool out1 = Mn[3] > 0.0f, in1 = interior && (Mn[3] < 0.0f);
+        if (out1 || in1) dt1 = (out1 ? (W0[3] + bq * a0h2) : (W1[3] + bq * a1h2)) * area * rcp(fabsf(Mn[3]) + LH_EPSF);
+    }
     if (!(dt <= dt0)) {
-        float al = dt0 * rcp(dt + LH_EPSF);
+        float al = dt0 * k.idt_eps;
         Mn[0] *= al; Mn[1] *= al; Mn[2] *= al;
         P[0] *= al; P[1] *= al; P[2] *= al;
     }
     if (!(dt <= dt1)) {
-        float al = dt1 * rcp(dt + LH_EPSF);
+        float al = dt1 * k.idt_eps;
         Mn[3] *= al; Mn[4] *= al; Mn[5] *= al;
         P[3] *= al; P[4] *= al; P[5] *= al;
     }
@@ -242,7 +251,7 @@ __device__ __forceinline__ float edge_dt0(const K32& k, const float* W0, const f
     float u1ij = (r00 * (w01 * sdes(w00, eps, eps4)) + r10 * (w11 * sdes(w10, eps, eps4))) * rcp(r00 + r10 + LH_EPSF);
     float u2ij = (r02 * (w03 * sdes(w02, eps, eps4)) + r12 * (w13 * sdes(w12, eps, eps4))) * rcp(r02 + r12 + LH_EPSF);
     float Dx, Dy, Dz, Dw;
-    eig<0>(h1ij, u1ij, h2ij, u2ij, k.g, k.gp, eps, eps4, Dx, Dy, Dz, Dw);
+    eig<0>(h1ij, u1ij, h2ij, u2ij, 0.0f, 0.0f, k.g, k.gp, k.igp, eps, eps4, Dx, Dy, Dz, Dw);
     float maxl = fmaxf(fmaxf(Dx, Dw), fmaxf(fabsf(u1ij), fabsf(u2ij)));
     if (maxl < eps) maxl += eps;
     return len * maxl;
@@ -294,6 +303,7 @@ __device__ __forceinline__ bool negative(Pair a) { return a.hi < 0.0f || (a.hi =
 
 struct KCell {
     float eps, eps4, r, g, mu, mfc, mf0, mfs, vmax1, vmax2, dtf, valf;
+    float ieps, ten_eps;
 };
 
 // Cell update (src/GPU/Volumen_kernel.cu:7-180, 231-284) on compensated
@@ -301,12 +311,9 @@ struct KCell {
 __device__ __forceinline__ void cell_update(const KCell& k, Pair* o, const float* acc)
 {
     const float eps = k.eps, eps4 = k.eps4, dt = k.dtf;
-    // old-state speeds for the implicit friction (from the old state)
+    // old-state velocities for the implicit friction (speeds taken where needed)
     const float so1 = sdes(o[0].hi, eps, eps4), so2 = sdes(o[3].hi, eps, eps4);
     const float uo1x = o[1].hi * so1, uo1y = o[2].hi * so1, uo2x = o[4].hi * so2, uo2y = o[5].hi * so2;
-    const float ddx = uo1x - uo2x, ddy = uo1y - uo2y;
-    const float du = fsqrt(ddx * ddx + ddy * ddy);
-    const float uo1 = fsqrt(uo1x * uo1x + uo1y * uo1y), uo2 = fsqrt(uo2x * uo2x + uo2y * uo2y);
 
     Pair n[6];
 #pragma unroll
@@ -316,14 +323,22 @@ __device__ __forceinline__ void cell_update(const KCell& k, Pair* o, const float
     if (negative(n[0])) n[0] = Pair{0.0f, 0.0f};
     if (negative(n[3])) n[3] = Pair{0.0f, 0.0f};
     {
-        float z = n[0].hi * rcp(eps);
-        float z2 = z * z;
-        float aux = fexp(-dt * rcp(z2 * z2 + LH_EPSF));
-        if (aux != 1.0f) { n[1] = scale(n[1], aux); n[2] = scale(n[2], aux); }
-        z = n[3].hi * rcp(eps);
-        z2 = z * z;
-        aux = fexp(-dt * rcp(z2 * z2 + LH_EPSF));
-        if (aux != 1.0f) { n[4] = scale(n[4], aux); n[5] = scale(n[5], aux); }
+        // exp(-dt / ((h/eps)^4 + EPS)) is exactly 1 in FP32 once the exponent is below 2^-26
+        const float lim = dt * 6.7108864e7f;
+        float z = n[0].hi * k.ieps;
+        float z4 = (z * z) * (z * z);
+        if (z4 < lim) {
+            float aux = fexp(-dt * rcp(z4 + LH_EPSF));
+            n[1] = scale(n[1], aux);
+            n[2] = scale(n[2], aux);
+        }
+        z = n[3].hi * k.ieps;
+        z4 = (z * z) * (z * z);
+        if (z4 < lim) {
+            float aux = fexp(-dt * rcp(z4 + LH_EPSF));
+            n[4] = scale(n[4], aux);
+            n[5] = scale(n[5], aux);
+        }
     }
     if (k.vmax1 > 0.0f) {
         float h = n[0].hi, s = sdes(h, eps, eps4);
@@ -348,6 +363,8 @@ __device__ __forceinline__ void cell_update(const KCell& k, Pair* o, const float
             phi = (hmod >= em) ? rcp(hmod) : SQRT2F * hmod * rsq(m2 * m2 + (em * em) * (em * em));
         }
         if ((h1 > 0.0f) && (h2 > 0.0f)) {
+            const float ddx = uo1x - uo2x, ddy = uo1y - uo2y;
+            const float du = fsqrt(ddx * ddx + ddy * ddy), uo2 = fsqrt(uo2x * uo2x + uo2y * uo2y);
             float c1 = dt * h2 * phi * k.mfc * du;
             float c2 = dt * r * h1 * phi * k.mfc * du;
             float c3 = dt * k.g * k.mfs * k.mfs * uo2 * rcp(__powf(h2, 4.0f / 3.0f) + LH_EPSF);
@@ -359,12 +376,14 @@ __device__ __forceinline__ void cell_update(const KCell& k, Pair* o, const float
             n[1] = q1x; n[4] = q2x; n[2] = q1y; n[5] = q2y;
         }
         if ((h1 > 0.0f) && (h2 < eps)) {
+            const float uo1 = fsqrt(uo1x * uo1x + uo1y * uo1y);
             float c = dt * k.g * k.mf0 * k.mf0 * uo1 * rcp(__powf(h1, 4.0f / 3.0f) + LH_EPSF);
             float f = h1 * s1 * rcp(1.0f + c);
             n[1] = scale(n[1], f);
             n[2] = scale(n[2], f);
         }
         if ((h2 > 0.0f) && (h1 < eps)) {
+            const float uo2 = fsqrt(uo2x * uo2x + uo2y * uo2y);
             float c = dt * k.g * k.mfs * k.mfs * uo2 * rcp(__powf(h2, 4.0f / 3.0f) + LH_EPSF);
             float f = h2 * s2 * rcp(1.0f + c);
             n[4] = scale(n[4], f);
@@ -374,8 +393,10 @@ __device__ __forceinline__ void cell_update(const KCell& k, Pair* o, const float
 
     // coulomb
     {
-        float x = 10.0f * n[0].hi * rcp(eps);
-        float fsc = 1.0f - k.r * (1.0f - fexp(-(x * x)));
+        float x = n[0].hi * k.ten_eps;
+        float ex = 0.0f;
+        if (x < 9.5f) ex = fexp(-(x * x));
+        float fsc = 1.0f - k.r * (1.0f - ex);
         float sc = fsc * k.mu * k.g * n[3].hi * dt;
         float nq = fsqrt(n[4].hi * n[4].hi + n[5].hi * n[5].hi);
         float f = ((nq + sc >= LH_EPSF) && (n[3].hi > 0.0f) && (nq >= sc)) ? nq * rcp(nq + sc) : 0.0f;

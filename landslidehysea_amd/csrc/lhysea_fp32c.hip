@@ -55,6 +55,10 @@ __device__ __forceinline__ K32 make_k32(const Consts& k, double dt)
     c.area = (float)k.area;
     c.len_x = (float)k.dy;
     c.len_y = (float)k.dx;
+    c.igp = (float)(1.0 / (k.g * (1.0 - k.r)));
+    c.ten_eps = (float)(10.0 / k.eps_h);
+    c.idt_eps = (float)(1.0 / ((double)c.dtf + (double)LH_EPSF));
+    c.bq = (float)(dt / k.area);
     return c;
 }
 
@@ -320,6 +324,8 @@ __global__ void __launch_bounds__(NT) k_fused(Consts k, const float* __restrict_
     kk.vmax2 = (float)k.vmax2;
     kk.dtf = kc.dtf;
     kk.valf = (float)(dt / k.area);
+    kk.ieps = (float)(1.0 / k.eps_h);
+    kk.ten_eps = kc.ten_eps;
     double Amax = 0.0;
     const double tnow = a.ctl->t;
     for (int e = tid; e < TX * TY; e += NT) {
