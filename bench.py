@@ -29,7 +29,9 @@ sys.path.insert(0, ROOT)
 from landslidehysea_amd import scenario as S          # noqa: E402
 from landslidehysea_amd import solver as L            # noqa: E402
 
-BYTES_PER_CELL = 104          # FP64: 6x8 read + 8 depth + 6x8 write; FP32c: 12x4 read+write... see DESIGN.md
+# algorithmic bytes per cell-update: FP64 reads 6x8 B state + 8 B depth and writes
+# 6x8 B; FP32c reads 6 values + 6 compensations (12x4 B) + 8 B depth and writes 12x4 B
+BYTES_PER_CELL = 104
 HBM_PEAK_GBS = 8000.0
 NX = NY = 2048
 SEED = 2048
@@ -79,6 +81,39 @@ def time_adjoint(precision, steps=4000, snaps=64, n=1024, ngauges=16):
             "reverse_ms": round(rev, 2), "cost_vs_forward": round((fwd + rev) / fwd, 3),
             "mcells_fwd_plus_adjoint": round(n * n * steps / ((fwd + rev) * 1e-3) / 1e6, 2),
             "re_advances": res["advances"], "wall_s": round(wall, 2)}
+
+
+def time_ensemble(steps=200, warmup=16, n=64, tune_steps=16):
+    """Ensemble workload: 64 seeded scenarios (sides U{256..1536}, one slide of
+    volume logU[1e6, 5e7] m^3, friction U[6, 16] deg, r U[0.45, 0.6]) stepped
+    together in FP32C by the batched kernel, the step loop replayed from a
+    captured graph; the tile shape is autotuned first."""
+    specs, build = S.make_ensemble(n)
+    params = []
+    for spec in specs:
+        params.append(L.Params(nx=spec["nx"], ny=spec["ny"], dx=S.DEFAULT_SPACING_DEG * S.DEG_TO_M,
+                               dy=S.DEFAULT_SPACING_DEG * S.DEG_TO_M, angle_deg=(spec["angle_deg"],),
+                               r=spec["r"], precision=L.FP32C))
+    cells = sum(p.nx * p.ny for p in params)
+    with L.Batch(params) as b:
+        for i, spec in enumerate(specs):
+            sc, _ = build(spec)
+            H, _ = S.hmin_shift(sc.H)
+            b.upload(i, np.ascontiguousarray(sc.state), np.ascontiguousarray(H))
+            del sc, H
+        b.initial_dt()
+        best, ms = b.autotune(tune_steps)
+        b.step(warmup)
+        b.sync()
+        total_ms = b.time_steps(steps)
+        t0 = [b.time(i) for i in (0, n - 1)]
+    shapes = L.batch_tile_shapes()
+    return {"scenarios": n, "cells": cells, "steps": steps, "total_ms": round(total_ms, 3),
+            "ms_per_step": round(total_ms / steps, 4), "mcells": round(cells * steps / (total_ms * 1e-3) / 1e6, 2),
+            "tile": list(shapes[best]), "autotune_ms": {f"{s[0]}x{s[1]}/{s[2]}": round(float(m), 3)
+                                                        for s, m in zip(shapes, ms)},
+            "achieved_gbs": round(BYTES_PER_CELL * cells * steps / (total_ms * 1e-3) / 1e9, 1),
+            "steps_done_first_last": [t0[0][2], t0[1][2]]}
 
 
 def cpu_baseline(sample_steps=3, nx=1024, ny=1024):
@@ -160,6 +195,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=4000,
                     help="steps of the adjoint workload (0 skips it)")
     ap.add_argument("--adjoint-snapshots", type=int, default=64)
+    ap.add_argument("--ensemble-steps", type=int, default=200,
+                    help="timed steps of the 64-scenario ensemble workload (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the whole grid lives in one MI355X's HBM); "
@@ -205,6 +242,8 @@ def main():
         if fp64:
             adj["fp64"] = time_adjoint(L.FP64, args.adjoint_steps, args.adjoint_snapshots)
         out["adjoint"] = adj
+    if args.ensemble_steps > 0 and fp32c:
+        out["ensemble"] = time_ensemble(args.ensemble_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

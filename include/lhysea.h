@@ -126,6 +126,55 @@ int lhysea_adjoint(lhysea_sim *s, int32_t nsteps, int32_t nsnap, const int32_t *
                    const double *obs, double *loss, double *grad_W0, double *grad_mu, double *grad_angle1,
                    double *stats);
 
+/* ---------------------------------------------------------------- ensembles
+ * A batch steps many independent FP32C scenarios (own grid, parameters, dt,
+ * time and stop condition) with one batched launch per step, replayed from a
+ * captured hipGraph.  No counterpart in the reference, which runs one
+ * scenario per process (src/GPU/main.cxx:157-164); each scenario's results
+ * equal those of a single lhysea_sim with the same inputs.  Scenario sid of a
+ * batch follows the single-simulation calls above, with the same meaning:
+ *   lhysea_batch_upload        <- lhysea_upload
+ *   lhysea_batch_download      <- lhysea_download
+ *   lhysea_batch_initial_dt    <- lhysea_initial_dt, for every scenario
+ *   lhysea_batch_get_time      <- lhysea_get_time
+ *   lhysea_batch_track_eta_max / lhysea_batch_eta_max <- lhysea_track_eta_max / lhysea_eta_max
+ *   lhysea_batch_set_gauges / lhysea_batch_get_gauges <- lhysea_set_gauges / lhysea_get_gauges
+ *   lhysea_batch_dt_history    <- lhysea_dt_history
+ * Every p[i] must have precision LHYSEA_FP32C. */
+typedef struct lhysea_batch lhysea_batch;
+
+int lhysea_batch_create(const lhysea_params *p, int32_t nscen, lhysea_batch **out);
+int lhysea_batch_destroy(lhysea_batch *b);
+int lhysea_batch_size(const lhysea_batch *b, int32_t *nscen);
+int lhysea_batch_upload(lhysea_batch *b, int32_t sid, const double *W, const double *H);
+int lhysea_batch_download(lhysea_batch *b, int32_t sid, double *W);
+int lhysea_batch_initial_dt(lhysea_batch *b);
+/* Scenario sid steps while t < t_end (the reference's loop condition,
+ * src/GPU/ShallowWater.cu:269) and steps < max_steps; INFINITY / -1: no limit. */
+int lhysea_batch_set_stop(lhysea_batch *b, int32_t sid, double t_end, int32_t max_steps);
+/* nsteps batch steps (graph replays of 16 steps plus direct launches), asynchronous */
+int lhysea_batch_step(lhysea_batch *b, int32_t nsteps);
+/* the same steps, every launch made directly (no graph) */
+int lhysea_batch_step_eager(lhysea_batch *b, int32_t nsteps);
+int lhysea_batch_sync(lhysea_batch *b);
+int lhysea_batch_get_time(lhysea_batch *b, int32_t sid, double *t, double *dt_next, int32_t *steps);
+int lhysea_batch_track_eta_max(lhysea_batch *b, int32_t sid, int enable);
+int lhysea_batch_eta_max(lhysea_batch *b, int32_t sid, double *val, double *time);
+int lhysea_batch_set_gauges(lhysea_batch *b, int32_t sid, const int32_t *cells, int32_t ng, int32_t max_steps);
+int lhysea_batch_get_gauges(lhysea_batch *b, int32_t sid, double *out, int32_t *nrec);
+int lhysea_batch_dt_history(lhysea_batch *b, int32_t sid, double *out, int32_t max_steps, int32_t *nrec);
+/* Tile shapes of the batched kernel: shapes gets 3 ints (TX, TY, threads) per
+ * shape (may be NULL), count the number of shapes. */
+int lhysea_batch_tile_shapes(int32_t *shapes, int32_t *count);
+int lhysea_batch_set_tile(lhysea_batch *b, int32_t tile);
+int lhysea_batch_get_tile(const lhysea_batch *b, int32_t *tile);
+/* HIP-event time of nsteps batch steps (they advance the state). */
+int lhysea_batch_time_steps(lhysea_batch *b, int32_t nsteps, double *total_ms);
+/* Time nsteps steps with every tile shape and keep the fastest; the state,
+ * times and outputs are left as they were.  ms_per_tile (one per shape) and
+ * best may be NULL. */
+int lhysea_batch_autotune(lhysea_batch *b, int32_t nsteps, double *ms_per_tile, int32_t *best);
+
 /* One-shot driver with shallowWater's contract: runs while t < t_end, writes
  * no files itself (output is the caller's, see L-HySEA.exe), returns the loop
  * wall time in *elapsed_s.  W is updated in place with the final state; eta

@@ -36,6 +36,8 @@ int hip_fail(hipError_t e, const char* where)
 
 }  // namespace
 
+void lhysea::set_last_error(const std::string& msg) { g_err = msg; }
+
 struct lhysea_sim {
     lhysea_params p{};
     Consts k{};
@@ -59,7 +61,7 @@ struct lhysea_sim {
     double* dt_seq = nullptr;    // frozen dt sequence (adjoint replay)
 };
 
-static Consts make_consts(const lhysea_params& p)
+Consts lhysea::make_consts(const lhysea_params& p)
 {
     Consts k{};
     k.nx = p.nx;

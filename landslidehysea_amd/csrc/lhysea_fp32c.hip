@@ -141,12 +141,20 @@ __global__ void k_finalize(Consts k, const float* __restrict__ Wn, const double*
 }
 
 // ----------------------------------------------------------------- fused step
+// One even-aligned TX x TY tile of one step (origin i0, j0): stage the tile
+// and its 1-cell halo, run the four edge classes in the reference's order,
+// write the compensated update of the tile's own cells.  Returns this
+// thread's max of the wave-speed sums A_i of the cells it updated.
 // LDS: sH (depth), sh1/sh2 (full-precision h as FP64), sq (q hi values),
 // sA (FP32 accumulators: dh1,dq1x,dq1y,dh2,dq2x,dq2y,A).
+// Every cell receives its class-C1..C4 contributions in that order whatever
+// the tile shape, so the result does not depend on TX, TY or NT.
 template <int TX, int TY, int NT>
-__global__ void __launch_bounds__(NT) k_fused(Consts k, const float* __restrict__ W, float* __restrict__ Wn,
-                                              const double* __restrict__ H, StepArgs a, int tiles_x, int tiles_y)
+__device__ __forceinline__ double tile_step(const Consts& k, const float* __restrict__ W, float* __restrict__ Wn,
+                                            const double* __restrict__ H, double dt, const Ctl* ctl,
+                                            double* __restrict__ emax, int i0, int j0)
 {
+    static_assert(TX % 2 == 0 && TY % 2 == 0, "tile origins must stay even");
     constexpr int CW = TX + 2, RW = TY + 2;
     __shared__ double sH[RW][CW];
     __shared__ double sh1[RW][CW];
@@ -156,16 +164,6 @@ __global__ void __launch_bounds__(NT) k_fused(Consts k, const float* __restrict_
 
     const int nx = k.nx, ny = k.ny;
     const size_t N = (size_t)nx * ny;
-    const int nb = tiles_x * tiles_y;
-    int b;
-    {
-        const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-        const int q = nb >> 3, rem = nb & 7;
-        b = xcd * q + (xcd < rem ? xcd : rem) + slot;
-    }
-    const int tx = b % tiles_x, ty = b / tiles_x;
-    const int i0 = tx * TX, j0 = ty * TY;
-    const double dt = step_dt(a.ctl, a.dt_seq);
     const K32 kc = make_k32(k, dt);
     const int tid = threadIdx.x;
 
@@ -327,7 +325,7 @@ __global__ void __launch_bounds__(NT) k_fused(Consts k, const float* __restrict_
     kk.ieps = (float)(1.0 / k.eps_h);
     kk.ten_eps = kc.ten_eps;
     double Amax = 0.0;
-    const double tnow = a.ctl->t;
+    const double tnow = ctl->t;
     for (int e = tid; e < TX * TY; e += NT) {
         const int r = e / TX, c = e % TX;
         const int gj = j0 + r, gi = i0 + c;
@@ -345,11 +343,11 @@ __global__ void __launch_bounds__(NT) k_fused(Consts k, const float* __restrict_
 #pragma unroll
         for (int f = 0; f < 6; f++) ac[f] = sA[f][r][c + 1];
         Amax = fmax(Amax, (double)sA[6][r][c + 1]);
-        if (a.emax) {
+        if (emax) {
             const double v = h1d - sH[r + 1][c + 1];
-            if (v > a.emax[2 * g]) {
-                a.emax[2 * g] = v;
-                a.emax[2 * g + 1] = tnow;
+            if (v > emax[2 * g]) {
+                emax[2 * g] = v;
+                emax[2 * g + 1] = tnow;
             }
         }
         cell_update(kk, o, ac);
@@ -359,10 +357,87 @@ __global__ void __launch_bounds__(NT) k_fused(Consts k, const float* __restrict_
             Wn[(6 + f) * N + g] = o[f].lo;
         }
     }
+    return Amax;
+}
+
+// Single-simulation step: one block per tile, tiles of one XCD contiguous
+// (blocks are dispatched round-robin over the 8 XCDs, so XCD x runs blocks
+// x, x+8, ...; giving it one run of tiles keeps shared halo rows in its L2).
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_fused(Consts k, const float* __restrict__ W, float* __restrict__ Wn,
+                                              const double* __restrict__ H, StepArgs a, int tiles_x, int tiles_y)
+{
+    const int nb = tiles_x * tiles_y;
+    int b;
+    {
+        const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+        const int q = nb >> 3, rem = nb & 7;
+        b = xcd * q + (xcd < rem ? xcd : rem) + slot;
+    }
+    const int tx = b % tiles_x, ty = b / tiles_x;
+    const double dt = step_dt(a.ctl, a.dt_seq);
+    const double Amax = tile_step<TX, TY, NT>(k, W, Wn, H, dt, a.ctl, a.emax, tx * TX, ty * TY);
     block_max_atomic<NT>(Amax, &a.ctl->maxA_next);
 }
 
 constexpr int FTX = 62, FTY = 14, FNT = 512;
+
+// ------------------------------------------------------------- ensemble step
+// One launch steps every scenario of a batch.  tab[blockIdx.x] = (scenario
+// << 20) | tile, built on the host with the same XCD-contiguous order over the
+// concatenated tile lists.  A scenario whose step budget or end time is
+// reached skips its tiles; its ping-pong parity is its own step count.
+__device__ __forceinline__ bool scen_active(const BatchScen& d, int step, double t)
+{
+    return step < d.max_steps && t < d.t_end;
+}
+
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_batch(const BatchScen* __restrict__ sd, const unsigned* __restrict__ tab)
+{
+    const unsigned e = tab[blockIdx.x];
+    const BatchScen& d = sd[e >> 20];
+    Ctl* ctl = d.ctl;
+    const int step = ctl->step;
+    if (!scen_active(d, step, ctl->t)) return;
+    const int tile = (int)(e & 0xFFFFFu);
+    const int tx = tile % d.tiles_x, ty = tile / d.tiles_x;
+    const int cur = step & 1;
+    const Consts k = d.k;
+    const double Amax = tile_step<TX, TY, NT>(k, d.W[cur], d.W[cur ^ 1], d.H, ctl->dt_cur, ctl, d.emax,
+                                              tx * TX, ty * TY);
+    block_max_atomic<NT>(Amax, &ctl->maxA_next);
+}
+
+// One block per scenario: gauges of the new state, dt history, t += dt, the
+// lagged dt of the next step from this step's wave-speed max.
+__global__ void k_batch_finalize(const BatchScen* __restrict__ sd)
+{
+    const BatchScen& d = sd[blockIdx.x];
+    Ctl* ctl = d.ctl;
+    const int s = ctl->step;
+    if (!scen_active(d, s, ctl->t)) return;
+    const double dt = ctl->dt_cur;
+    const size_t N = (size_t)d.k.nx * d.k.ny;
+    const float* Wn = d.W[(s & 1) ^ 1];
+    if (d.gseries && s < d.gcap)
+        for (int g = threadIdx.x; g < d.ng; g += blockDim.x) {
+            const size_t c = (size_t)d.gauges[g];
+            const double h1 = (double)Wn[c] + (double)Wn[6 * N + c];
+            const double h2 = (double)Wn[3 * N + c] + (double)Wn[9 * N + c];
+            d.gseries[(size_t)s * d.ng + g] = h1 + h2 - d.H[c];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (d.dt_hist && s < d.hcap) d.dt_hist[s] = dt;
+        const double maxA = __longlong_as_double((long long)ctl->maxA_next);
+        ctl->t = ctl->t + dt;
+        ctl->dt_cur = dt_from_maxA(maxA, d.k.cfl, d.k.area);
+        ctl->maxA_last = maxA;
+        ctl->maxA_next = 0ull;
+        ctl->step = s + 1;
+    }
+}
 
 }  // namespace f32
 
@@ -384,6 +459,21 @@ hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const dou
                        dim3(f32::FNT), 0, s, k, W, Wn, H, a, tiles_x, tiles_y);
     if (a.ev1) hipEventRecord(a.ev1, s);
     hipLaunchKernelGGL(f32::k_finalize, dim3(1), dim3(64), 0, s, k, Wn, H, a);
+    return hipGetLastError();
+}
+
+const BatchTile kBatchTiles[kNumBatchTiles] = {{62, 14, 512}, {30, 14, 256}, {62, 30, 1024}, {126, 14, 1024}};
+
+hipError_t f32c_batch_step(int tile, const BatchScen* sd, int nscen, const unsigned* tab, int nblocks, hipStream_t s)
+{
+    switch (tile) {
+    case 0: hipLaunchKernelGGL((f32::k_batch<62, 14, 512>), dim3(nblocks), dim3(512), 0, s, sd, tab); break;
+    case 1: hipLaunchKernelGGL((f32::k_batch<30, 14, 256>), dim3(nblocks), dim3(256), 0, s, sd, tab); break;
+    case 2: hipLaunchKernelGGL((f32::k_batch<62, 30, 1024>), dim3(nblocks), dim3(1024), 0, s, sd, tab); break;
+    case 3: hipLaunchKernelGGL((f32::k_batch<126, 14, 1024>), dim3(nblocks), dim3(1024), 0, s, sd, tab); break;
+    default: return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(f32::k_batch_finalize, dim3(nscen), dim3(64), 0, s, sd);
     return hipGetLastError();
 }
 

@@ -7,7 +7,13 @@
 #include <vector>
 #include "lhysea_common.hpp"
 
+struct lhysea_params;
+
 namespace lhysea {
+
+// Shared by the C-ABI files (lhysea_capi.hip)
+Consts make_consts(const lhysea_params& p);
+void set_last_error(const std::string& msg);
 
 enum Precision { PREC_FP64 = 0, PREC_FP32C = 1 };
 enum KernelMode { MODE_FOURPASS = 0, MODE_FUSED = 1 };
@@ -50,6 +56,30 @@ hipError_t f64_step_fused(const Consts& k, const double* W, double* Wn, const do
 // FP32-compensated launchers (lhysea_fp32c.hip)
 hipError_t f32c_initial_dt(const Consts& k, const float* W, Ctl* ctl, hipStream_t s);
 hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const double* H, const StepArgs& a,
+                           hipStream_t s);
+
+// One scenario of an ensemble batch as the batched kernels read it (device copy).
+struct BatchScen {
+    Consts k;
+    const double* H;        // N
+    float* W[2];            // ping-pong, 12 planes of N; the current one is W[ctl->step & 1]
+    Ctl* ctl;
+    double* emax;           // 2N or null
+    const int* gauges;
+    double* gseries;        // row r holds step r (base already offset)
+    double* dt_hist;
+    double t_end;           // a scenario steps while t < t_end and step < max_steps
+    int ng, gcap, hcap, max_steps;
+    int tiles_x, tiles_y;
+};
+
+// Tile shapes of the batched kernel (TX x TY cells, NT threads)
+struct BatchTile {
+    int tx, ty, nt;
+};
+constexpr int kNumBatchTiles = 4;
+extern const BatchTile kBatchTiles[kNumBatchTiles];
+hipError_t f32c_batch_step(int tile, const BatchScen* sd, int nscen, const unsigned* tab, int nblocks,
                            hipStream_t s);
 
 // Adjoint (lhysea_adjoint.hip)

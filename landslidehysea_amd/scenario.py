@@ -158,3 +158,39 @@ def hmin_shift(H: np.ndarray) -> tuple[np.ndarray, float]:
     if hmin >= 0.0:
         return H.copy(), 0.0
     return H - hmin, hmin
+
+
+def make_ensemble(n: int = 64, seed: int = 64, side: tuple = (256, 1536), volume: tuple = (1e6, 5e7),
+                  angle_deg: tuple = (6.0, 16.0), r: tuple = (0.45, 0.6), semi_axes: tuple = (10.0, 30.0)):
+    """A seeded ensemble of slide scenarios for batched runs.
+
+    Scenario k: grid sides nx, ny ~ U{side}, its own seeded bathymetry, one
+    paraboloid slide with semi-axes ~ U[semi_axes] cells centred at
+    (U[0.3, 0.6] nx, U[0.3, 0.7] ny), thickness set by a volume
+    V = (pi/2) hmax a b dx dy with log V ~ U[log volume], friction angle
+    ~ U[angle_deg] and density ratio r ~ U[r].  Returns (specs, build): the
+    drawn parameters of every scenario, and build(spec) -> (Scenario,
+    {"angle_deg": ..., "r": ...}) that makes one scenario's arrays at a time."""
+    rng = SplitMix64(seed)
+    specs = []
+    for k in range(n):
+        nx = rng.randint(*side)
+        ny = rng.randint(*side)
+        a = rng.uniform(*semi_axes)
+        b = rng.uniform(*semi_axes)
+        i0 = rng.uniform(0.3, 0.6) * nx
+        j0 = rng.uniform(0.3, 0.7) * ny
+        vol = math.exp(rng.uniform(math.log(volume[0]), math.log(volume[1])))
+        ang = rng.uniform(*angle_deg)
+        rr = rng.uniform(*r)
+        specs.append(dict(nx=nx, ny=ny, seed=(seed << 16) + k, a=a, b=b, i0=i0, j0=j0, volume=vol,
+                          angle_deg=ang, r=rr))
+
+    def build(spec):
+        dx = dy = DEFAULT_SPACING_DEG * DEG_TO_M
+        hmax = spec["volume"] / (0.5 * math.pi * spec["a"] * spec["b"] * dx * dy)
+        slide = Slide(spec["i0"], spec["j0"], hmax, spec["a"], spec["b"])
+        sc = make_scenario(spec["nx"], spec["ny"], spec["seed"], slides=[slide])
+        return sc, {"angle_deg": spec["angle_deg"], "r": spec["r"]}
+
+    return specs, build

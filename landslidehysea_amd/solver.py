@@ -122,6 +122,27 @@ SIGNATURES = {
     "lhysea_adjoint": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, C.c_int32, _dp, _dp, _dp, _dp, _dp,
                                  _dp]),
     "lhysea_shallow_water": (C.c_int, [C.POINTER(CParams), _dp, _dp, C.c_double, _dp, _dp, _dp, _ip]),
+    "lhysea_batch_create": (C.c_int, [C.POINTER(CParams), C.c_int32, C.POINTER(C.c_void_p)]),
+    "lhysea_batch_destroy": (C.c_int, [C.c_void_p]),
+    "lhysea_batch_size": (C.c_int, [C.c_void_p, _ip]),
+    "lhysea_batch_upload": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
+    "lhysea_batch_download": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    "lhysea_batch_initial_dt": (C.c_int, [C.c_void_p]),
+    "lhysea_batch_set_stop": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_int32]),
+    "lhysea_batch_step": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lhysea_batch_step_eager": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lhysea_batch_sync": (C.c_int, [C.c_void_p]),
+    "lhysea_batch_get_time": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _ip]),
+    "lhysea_batch_track_eta_max": (C.c_int, [C.c_void_p, C.c_int32, C.c_int]),
+    "lhysea_batch_eta_max": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
+    "lhysea_batch_set_gauges": (C.c_int, [C.c_void_p, C.c_int32, _ip, C.c_int32, C.c_int32]),
+    "lhysea_batch_get_gauges": (C.c_int, [C.c_void_p, C.c_int32, _dp, _ip]),
+    "lhysea_batch_dt_history": (C.c_int, [C.c_void_p, C.c_int32, _dp, C.c_int32, _ip]),
+    "lhysea_batch_tile_shapes": (C.c_int, [_ip, _ip]),
+    "lhysea_batch_set_tile": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lhysea_batch_get_tile": (C.c_int, [C.c_void_p, _ip]),
+    "lhysea_batch_time_steps": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    "lhysea_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, _dp, _ip]),
 }
 
 
@@ -266,6 +287,130 @@ class Simulation:
         return {"loss": loss.value, "grad_W0": gW, "grad_mu": gmu.value, "grad_angle1": gang.value,
                 "forward_ms": stats[0], "reverse_ms": stats[1], "advances": int(stats[2]),
                 "reverse_steps": int(stats[3]), "snapshots": int(stats[4]), "optimal_advances": int(stats[5])}
+
+
+def batch_tile_shapes() -> list:
+    """(TX, TY, threads) of each tile shape the batched kernel offers."""
+    lib = load_library()
+    n = C.c_int32()
+    _check(lib.lhysea_batch_tile_shapes(None, C.byref(n)))
+    a = np.zeros(3 * n.value, dtype=np.int32)
+    _check(lib.lhysea_batch_tile_shapes(a.ctypes.data_as(_ip), C.byref(n)))
+    return [tuple(int(v) for v in a[3 * i:3 * i + 3]) for i in range(n.value)]
+
+
+class Batch:
+    """An ensemble of independent FP32C scenarios stepped together (a batch
+    handle of the C-ABI).  Scenario ``sid`` offers the calls of Simulation."""
+
+    def __init__(self, params: list):
+        self.params = list(params)
+        self.lib = load_library()
+        arr = (CParams * len(self.params))(*[p.to_c() for p in self.params])
+        self._cp = arr
+        h = C.c_void_p()
+        _check(self.lib.lhysea_batch_create(arr, len(self.params), C.byref(h)))
+        self.h = h
+        self._gshape = {}
+
+    def __len__(self):
+        return len(self.params)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lhysea_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, sid: int, W: np.ndarray, H: np.ndarray):
+        p = self.params[sid]
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(6, -1)
+        H = np.ascontiguousarray(H, dtype=np.float64).reshape(-1)
+        assert W.shape[1] == p.nx * p.ny and H.size == p.nx * p.ny
+        _check(self.lib.lhysea_batch_upload(self.h, sid, _dptr(W), _dptr(H)))
+
+    def download(self, sid: int) -> np.ndarray:
+        p = self.params[sid]
+        W = np.zeros((6, p.ny, p.nx))
+        _check(self.lib.lhysea_batch_download(self.h, sid, _dptr(W)))
+        return W
+
+    def initial_dt(self):
+        _check(self.lib.lhysea_batch_initial_dt(self.h))
+
+    def set_stop(self, sid: int, t_end: float = math.inf, max_steps: int = -1):
+        _check(self.lib.lhysea_batch_set_stop(self.h, sid, t_end, max_steps))
+
+    def step(self, n: int = 1, graph: bool = True):
+        fn = self.lib.lhysea_batch_step if graph else self.lib.lhysea_batch_step_eager
+        _check(fn(self.h, n))
+
+    def sync(self):
+        _check(self.lib.lhysea_batch_sync(self.h))
+
+    def time(self, sid: int):
+        t, d, n = C.c_double(), C.c_double(), C.c_int32()
+        _check(self.lib.lhysea_batch_get_time(self.h, sid, C.byref(t), C.byref(d), C.byref(n)))
+        return t.value, d.value, n.value
+
+    def track_eta_max(self, sid: int, on: bool = True):
+        _check(self.lib.lhysea_batch_track_eta_max(self.h, sid, 1 if on else 0))
+
+    def eta_max(self, sid: int):
+        p = self.params[sid]
+        v = np.zeros((p.ny, p.nx))
+        t = np.zeros_like(v)
+        _check(self.lib.lhysea_batch_eta_max(self.h, sid, _dptr(v), _dptr(t)))
+        return v, t
+
+    def set_gauges(self, sid: int, cells, max_steps: int):
+        c = np.ascontiguousarray(cells, dtype=np.int32)
+        _check(self.lib.lhysea_batch_set_gauges(self.h, sid, c.ctypes.data_as(_ip), len(c), max_steps))
+        self._gshape[sid] = (max_steps, len(c))
+
+    def gauges(self, sid: int) -> np.ndarray:
+        out = np.zeros(self._gshape.get(sid, (0, 0)))
+        n = C.c_int32()
+        _check(self.lib.lhysea_batch_get_gauges(self.h, sid, _dptr(out) if out.size else None, C.byref(n)))
+        return out[: n.value]
+
+    def dt_history(self, sid: int, max_steps: int) -> np.ndarray:
+        out = np.zeros(max_steps)
+        n = C.c_int32()
+        _check(self.lib.lhysea_batch_dt_history(self.h, sid, _dptr(out), max_steps, C.byref(n)))
+        return out[: n.value]
+
+    def set_tile(self, tile: int):
+        _check(self.lib.lhysea_batch_set_tile(self.h, tile))
+
+    def tile(self) -> int:
+        t = C.c_int32()
+        _check(self.lib.lhysea_batch_get_tile(self.h, C.byref(t)))
+        return t.value
+
+    def time_steps(self, nsteps: int) -> float:
+        ms = C.c_double()
+        _check(self.lib.lhysea_batch_time_steps(self.h, nsteps, C.byref(ms)))
+        return ms.value
+
+    def autotune(self, nsteps: int = 20):
+        """Time every tile shape and keep the fastest; returns (best, ms per shape)."""
+        n = len(batch_tile_shapes())
+        ms = np.zeros(n)
+        best = C.c_int32()
+        _check(self.lib.lhysea_batch_autotune(self.h, nsteps, _dptr(ms), C.byref(best)))
+        return best.value, ms
 
 
 def shallow_water(params: Params, W: np.ndarray, H: np.ndarray, t_end: float):

@@ -40,3 +40,17 @@ def test_probe_without_gpu_reports_no_device():
     from landslidehysea_amd import solver
     rc = solver.device_probe()
     assert rc in (0, 1, 2)
+
+
+def test_batch_rejects_invalid_scenarios_before_touching_a_device():
+    """Argument checks of lhysea_batch_create run before any device call."""
+    from landslidehysea_amd import solver
+    p = solver.Params(nx=8, ny=8, dx=1.0, dy=1.0, precision=solver.FP64)
+    with pytest.raises(solver.LhyseaError) as e:
+        solver.Batch([p])
+    assert e.value.code == 3 and "FP32C" in str(e.value)
+    q = solver.Params(nx=8, ny=8, dx=1.0, dy=1.0, precision=solver.FP32C, friction_law=1)
+    with pytest.raises(solver.LhyseaError) as e:
+        solver.Batch([q])
+    assert e.value.code == 3
+    assert len(solver.batch_tile_shapes()) >= 2

@@ -1,0 +1,184 @@
+"""Ensemble batch (GPU): many FP32C scenarios in one launch per step.
+
+Bar: bit-exact.  Each scenario of a batch must reproduce, bit for bit, the
+single-simulation FP32C path on the same inputs (state, time, dt, gauges, eta
+max), whatever the tile shape, whether the steps come from graph replays or
+direct launches, and whatever the other scenarios do (stop conditions).  The
+single-simulation path itself is checked against the FP64 oracle in
+test_gpu_fp32c.py.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests.cases import case
+from landslidehysea_amd import solver as L
+from landslidehysea_amd.scenario import gauge_cells
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+SPECS = [
+    dict(nx=50, ny=40, seed=1, angle=2.0, semi_axes=(8.0, 6.0)),
+    dict(nx=64, ny=48, seed=2, angle=6.0, semi_axes=(10.0, 8.0), borde=(1.0, 1.0, -1.0, 1.0), r=0.55),
+    dict(nx=37, ny=23, seed=3, angle=1.0, semi_axes=(6.0, 5.0), cfl=0.4),
+    dict(nx=130, ny=17, seed=4, angle=3.0, semi_axes=(12.0, 4.0)),
+]
+
+
+def build(specs):
+    out = []
+    for s in specs:
+        s = dict(s)
+        nx, ny = s.pop("nx"), s.pop("ny")
+        sc, W, H, prm = case(nx, ny, precision=L.FP32C, **s)
+        out.append((sc, W, H, prm))
+    return out
+
+
+def single(sc, W, H, prm, steps, gauges=None):
+    with L.Simulation(prm) as sim:
+        sim.upload(W, H)
+        sim.track_eta_max(True)
+        if gauges is not None:
+            sim.set_gauges(gauges, steps)
+        sim.step(steps)
+        res = {"W": sim.download(), "time": sim.time(), "emax": sim.eta_max(), "dt": sim.dt_history(steps)}
+        if gauges is not None:
+            res["gauges"] = sim.gauges()
+    return res
+
+
+def same(a, b):
+    return np.array_equal(np.asarray(a), np.asarray(b))
+
+
+def test_batch_equals_single_runs_bitwise():
+    cases = build(SPECS)
+    steps = 40                      # two graph replays of 16 steps + 8 direct launches
+    say(f"batch of {len(cases)} scenarios, {steps} steps")
+    gauges = [gauge_cells(sc, 4, seed=7) for sc, _, _, _ in cases]
+    with L.Batch([c[3] for c in cases]) as b:
+        for i, (sc, W, H, prm) in enumerate(cases):
+            b.upload(i, W, H)
+            b.track_eta_max(i, True)
+            b.set_gauges(i, gauges[i], steps)
+        b.step(steps)
+        got = [dict(W=b.download(i), time=b.time(i), emax=b.eta_max(i), gauges=b.gauges(i),
+                    dt=b.dt_history(i, steps)) for i in range(len(cases))]
+    for i, (sc, W, H, prm) in enumerate(cases):
+        ref = single(sc, W, H, prm, steps, gauges[i])
+        say(f"scenario {i} {prm.nx}x{prm.ny}: t {got[i]['time'][0]:.6f} vs {ref['time'][0]:.6f}")
+        assert got[i]["time"] == ref["time"]
+        assert same(got[i]["dt"], ref["dt"])
+        assert same(got[i]["W"], ref["W"])
+        assert same(got[i]["emax"][0], ref["emax"][0]) and same(got[i]["emax"][1], ref["emax"][1])
+        assert got[i]["gauges"].shape == (steps, 4)
+        assert same(got[i]["gauges"], ref["gauges"])
+    # the scenarios moved
+    assert np.abs(got[0]["W"][4]).max() > 0.0
+
+
+def test_batch_result_is_independent_of_tile_shape():
+    cases = build(SPECS[:3])
+    shapes = L.batch_tile_shapes()
+    say("tile shapes", shapes)
+    steps = 24
+    results = []
+    for t in range(len(shapes)):
+        with L.Batch([c[3] for c in cases]) as b:
+            b.set_tile(t)
+            assert b.tile() == t
+            for i, (sc, W, H, prm) in enumerate(cases):
+                b.upload(i, W, H)
+            b.step(steps)
+            results.append([b.download(i) for i in range(len(cases))])
+        say(f"tile {shapes[t]} done")
+    for t in range(1, len(shapes)):
+        for i in range(len(cases)):
+            assert same(results[t][i], results[0][i]), (shapes[t], i)
+
+
+def test_batch_stop_conditions():
+    cases = build(SPECS[:3])
+    ref = [single(sc, W, H, prm, 30) for sc, W, H, prm in cases]
+    # scenario 0: step budget 10; scenario 1: end time reached after exactly 15
+    # steps (t_end = its time after 15 steps: the loop runs while t < t_end);
+    # scenario 2: no limit
+    dt1 = ref[1]["dt"]
+    t15 = 0.0
+    for d in dt1[:15]:
+        t15 = t15 + d
+    with L.Batch([c[3] for c in cases]) as b:
+        for i, (sc, W, H, prm) in enumerate(cases):
+            b.upload(i, W, H)
+        b.set_stop(0, max_steps=10)
+        b.set_stop(1, t_end=t15)
+        b.step(30)
+        times = [b.time(i) for i in range(3)]
+        states = [b.download(i) for i in range(3)]
+    say("steps done", [t[2] for t in times])
+    assert times[0][2] == 10 and times[1][2] == 15 and times[2][2] == 30
+    assert times[1][0] == t15
+    for i, n in ((0, 10), (1, 15)):
+        sc, W, H, prm = cases[i]
+        r = single(sc, W, H, prm, n)
+        assert same(states[i], r["W"]) and times[i] == r["time"]
+    assert same(states[2], ref[2]["W"])
+
+
+def test_batch_graph_replays_match_direct_launches():
+    """Graph replays interleaved with host work (downloads, queries) and
+    direct launches give the same states as direct launches only."""
+    cases = build(SPECS[:2])
+    with L.Batch([c[3] for c in cases]) as bg, L.Batch([c[3] for c in cases]) as be:
+        for i, (sc, W, H, prm) in enumerate(cases):
+            bg.upload(i, W, H)
+            be.upload(i, W, H)
+        for rep in range(4):
+            bg.step(16)                        # one replay
+            bg.step(3)                         # direct launches
+            be.step(19, graph=False)
+            for i in range(len(cases)):
+                a, e = bg.download(i), be.download(i)
+                assert same(a, e), (rep, i)
+                assert bg.time(i) == be.time(i)
+            say(f"replay round {rep} equal")
+        bg.step(48)                            # three replays back to back
+        be.step(48, graph=False)
+        for i in range(len(cases)):
+            assert same(bg.download(i), be.download(i))
+
+
+def test_batch_autotune_keeps_state():
+    cases = build(SPECS[:3])
+    with L.Batch([c[3] for c in cases]) as b:
+        for i, (sc, W, H, prm) in enumerate(cases):
+            b.upload(i, W, H)
+        b.step(5)
+        before = [(b.download(i), b.time(i)) for i in range(3)]
+        best, ms = b.autotune(4)
+        say("autotune", best, ms)
+        assert 0 <= best < len(ms) and np.all(ms > 0)
+        assert b.tile() == best
+        for i in range(3):
+            assert same(b.download(i), before[i][0]) and b.time(i) == before[i][1]
+        b.step(7)
+        after = [b.download(i) for i in range(3)]
+    for i, (sc, W, H, prm) in enumerate(cases):
+        assert same(after[i], single(sc, W, H, prm, 12)["W"])
+
+
+def test_batch_timing_reports_time():
+    cases = build(SPECS[:2])
+    with L.Batch([c[3] for c in cases]) as b:
+        for i, (sc, W, H, prm) in enumerate(cases):
+            b.upload(i, W, H)
+        ms = b.time_steps(20)
+        assert ms > 0 and math.isfinite(ms)
+        assert b.time(0)[2] == 20
