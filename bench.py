@@ -33,6 +33,11 @@ from landslidehysea_amd import solver as L            # noqa: E402
 # 6x8 B; FP32c reads 6 values + 6 compensations (12x4 B) + 8 B depth and writes 12x4 B
 BYTES_PER_CELL = 104
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the fused FP32C step kernel at 2048x2048, measured with
+# rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the scale factors of a plain
+# 256 MiB copy (profiles/fused_step_fp32c_2048x2048_counters.txt,
+# profiles/copy_256MiB_counter_calibration.txt)
+TRAFFIC_BYTES_FP32C = 440.3e6
 NX = NY = 2048
 SEED = 2048
 
@@ -171,7 +176,7 @@ def summarize(dbs):
             rows = c.execute("select kernel_name, counter_name, count(*), avg(value) from counters_collection "
                              "group by kernel_name, counter_name").fetchall()
             if rows:
-                print("kernel | counter | dispatches | avg_value (KiB)")
+                print("kernel | counter | dispatches | avg_value (KiB for FETCH_SIZE/WRITE_SIZE, else counts)")
                 for name, cn, n, v in rows:
                     print(f"{name[:110]} | {cn} | {n} | {v:.1f}")
         except sqlite3.Error:
@@ -232,7 +237,8 @@ def main():
         "speedup_vs_fp64": round(fp32c["mcells"] / fp64["mcells"], 3) if (fp32c and fp64) else None,
         "fp64_kernel_ms": round(fp64["kernel_ms"], 5) if fp64 else None,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": TRAFFIC_BYTES_FP32C if head is fp32c else None,
                      "kernel_ms": round(head["kernel_ms"], 5), "bytes_per_cell": BYTES_PER_CELL},
     }
     if args.adjoint_steps > 0:

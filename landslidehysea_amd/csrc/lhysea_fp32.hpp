@@ -239,6 +239,136 @@ __device__ __forceinline__ bool edge(const K32& k, const float* W0, const float*
     return true;
 }
 
+// Sorting network of eig() for a state whose internal pair is zero
+// (Dy = Dz = 0): returns the network's Dx (the minimum) and Dw outputs and
+// max(|Dy|, |Dz|) of its final Dy, Dz.
+__device__ __forceinline__ void net_zero_pair(float& Dx, float& Dw, float* aut2)
+{
+    float Dy = 0.0f, Dz = 0.0f, lo, hi;
+    lo = fminf(Dx, Dy); hi = fmaxf(Dx, Dy); Dx = lo; Dy = hi;
+    lo = fminf(Dx, Dz); hi = fmaxf(Dx, Dz); Dx = lo; Dz = hi;
+    lo = fminf(Dx, Dw); hi = fmaxf(Dx, Dw); Dx = lo; Dw = hi;
+    lo = fminf(Dy, Dz); hi = fmaxf(Dy, Dz); Dy = lo; Dz = hi;
+    lo = fminf(Dy, Dw); hi = fmaxf(Dy, Dw); Dy = lo; Dw = hi;
+    lo = fminf(Dy, Dz); hi = fmaxf(Dy, Dz); Dy = hi; Dz = lo;   // inverted (2,1) step
+    lo = fminf(Dz, Dw); hi = fmaxf(Dz, Dw); Dz = lo; Dw = hi;
+    if (aut2) *aut2 = fmaxf(fabsf(Dy), fabsf(Dz));
+}
+
+// edge() for an edge whose two cells carry no sediment (h2 = 0 and q2 = 0 on
+// both sides, h2 exactly 0 in FP64 too).  Every term of edge() that such an
+// edge makes zero is dropped and the rest is evaluated in the same order, so
+// the fluxes equal edge()'s up to the sign of zero results; the layer-2
+// fluxes are zero and are not produced (Fm[3..5], Fp[3..5] left unset):
+//  * dry/wet: the layer-2 tests only zero momenta that are already 0, the
+//    mixed tests reduce to the layer-1 ones;
+//  * h2ij = u2ij = 0, so tp3, the Coulomb flag (false), deta2 (0) and the
+//    internal eigenvalue pair (0, 0) vanish; the PVM takes the two-wave or
+//    Rusanov form because h2ij < eps;
+//  * eig(FLAG 1) of a cell uses s(f0 + f2) = s(w) of that cell's water.
+__device__ __forceinline__ bool edge1(const K32& k, const float* W0, const float* W1, double h10, double h11,
+                                      double H0, double H1, float n1x, float n1y, float len, float a0h1,
+                                      float a1h1, bool interior, EdgeFlux& out)
+{
+    const float eps = k.eps, eps4 = k.eps4, g = k.g;
+    float w00 = W0[0], w01 = W0[1] * n1x + W0[2] * n1y;
+    float w10 = W1[0], w11 = W1[1] * n1x + W1[2] * n1y;
+
+    // dry/wet treatment (src/GPU/Arista_kernel.cu:301-321) with h2 = 0
+    if ((w00 < eps) && (H1 - h11 > H0)) w11 = 0.0f;
+    if ((w10 < eps) && (H0 - h10 > H1)) w01 = 0.0f;
+
+    float h1ij = 0.5f * (w00 + w10);
+    float s00 = sdes(w00, eps, eps4), s10 = sdes(w10, eps, eps4);
+    float r00 = fsqrt(w00), r10 = fsqrt(w10);
+    float u1ij = (r00 * (w01 * s00) + r10 * (w11 * s10)) * rcp(r00 + r10 + LH_EPSF);
+    if (!(h1ij >= LH_EPSF)) return false;
+
+    // hydrostatic reconstruction in FP64 (src/GPU/Arista_kernel.cu:94-150)
+    const double Hm = fmin(H0, H1);
+    const double e10 = fmax(h10 - H0 + Hm, 0.0), e11 = fmax(h11 - H1 + Hm, 0.0);
+    const float deta1 = (float)(e11 - e10);
+    float tp1 = g * h1ij * deta1;
+
+    // averaged state, then both cells, internal pair zero
+    float Dx, Dw, aut2, Lx, Lw, Rx, Rw;
+    {
+        float sh = sdes(h1ij, eps, eps4);
+        float u = (h1ij * u1ij) * sh;
+        float cg = fsqrt(g * h1ij);
+        Dx = u - cg;
+        Dw = u + cg;
+        net_zero_pair(Dx, Dw, &aut2);
+    }
+    {
+        float u = w01 * s00, cg = fsqrt(g * w00);
+        Lx = u - cg;
+        Lw = u + cg;
+        net_zero_pair(Lx, Lw, nullptr);
+        u = w11 * s10;
+        cg = fsqrt(g * w10);
+        Rx = u - cg;
+        Rw = u + cg;
+        net_zero_pair(Rx, Rw, nullptr);
+    }
+    float aut1 = fminf(Dx, fminf(Lx, Rx));
+    float aut3 = fmaxf(Dw, fmaxf(Lw, Rw));
+    float maxl = fmaxf(fmaxf(fabsf(aut3), fabsf(aut1)), fabsf(u1ij));
+    float sg = aut1 + aut3;
+    if (!(fabsf(sg) < LH_EPSF) && !(sg > 0.0f)) aut2 = -aut2;
+
+    // PVM coefficients (src/GPU/Arista_kernel.cu:446-465), h2ij < eps
+    float c0, c1;
+    float d13 = aut1 - aut3, d12 = aut1 - aut2, d23 = aut2 - aut3;
+    if ((fabsf(d13) >= LH_EPSF) && (fabsf(d12) >= LH_EPSF) && (fabsf(d23) >= LH_EPSF)) {
+        float id = rcp(-d13);
+        c0 = (aut3 * fabsf(aut1) - aut1 * fabsf(aut3)) * id;
+        c1 = (fabsf(aut3) - fabsf(aut1)) * id;
+    } else {
+        c0 = maxl;
+        c1 = 0.0f;
+    }
+
+    float FL1 = (w00 < LH_EPSF) ? 0.0f : w01 * w01 * s00;
+    float FR1 = (w10 < LH_EPSF) ? 0.0f : w11 * w11 * s10;
+    float dF0 = w11 - w01, dF1 = FR1 - FL1;
+    float t20 = dF0, t21 = tp1 + dF1;
+    float I0 = deta1;
+    float V0 = 0.5f * (c0 * I0 + c1 * t20);
+    float V1 = 0.5f * (c0 * dF0 + c1 * t21);
+    float M0 = 0.5f * dF0, M1 = 0.5f * (dF1 + tp1);
+    float Fm0 = M0 + (w01 - V0), Fm1 = M1 + (FL1 - V1);
+    float Fp1 = M1 + (V1 - FR1);
+
+    // tangential upwinding (src/GPU/Arista_kernel.cu:506-532)
+    float ut1 = 0.0f;
+    if (!(fabsf(Fm0) < LH_EPSF)) {
+        ut1 = (Fm0 > 0.0f) ? (W0[2] * n1x - W0[1] * n1y) * s00 : (W1[2] * n1x - W1[1] * n1y) * s10;
+    }
+    float T1 = Fm0 * ut1;
+    float* P = out.Fp;
+    float* Mn = out.Fm;
+    Mn[0] = len * Fm0; Mn[1] = len * (Fm1 * n1x - T1 * n1y); Mn[2] = len * (Fm1 * n1y + T1 * n1x);
+    P[0] = -Mn[0];
+    P[1] = len * (Fp1 * n1x + T1 * n1y); P[2] = len * (Fp1 * n1y - T1 * n1x);
+
+    // positivity with the partial accumulators (src/GPU/Arista_kernel.cu:554-609)
+    const float area = k.area, dt = k.dtf, bq = k.bq;
+    float dt0 = 1e30f;
+    {
+        const bool out0 = Mn[0] > 0.0f, in0 = interior && (Mn[0] < 0.0f);
+        if (out0 || in0) dt0 = (out0 ? (W0[0] + bq * a0h1) : (W1[0] + bq * a1h1)) * area * rcp(fabsf(Mn[0]) + LH_EPSF);
+    }
+    if (!(dt <= dt0)) {
+        float al = dt0 * k.idt_eps;
+        Mn[0] *= al; Mn[1] *= al; Mn[2] *= al;
+        P[0] *= al; P[1] *= al; P[2] *= al;
+    }
+    if (maxl < eps) maxl += eps;
+    out.lam = len * maxl * rcp(k.peso);
+    return true;
+}
+
 // Initial wave-speed edge (src/GPU/Arista_kernel.cu:1040-1127) in FP32.
 __device__ __forceinline__ float edge_dt0(const K32& k, const float* W0, const float* W1, float n1x, float n1y,
                                           float len)
@@ -404,6 +534,52 @@ __device__ __forceinline__ void cell_update(const KCell& k, Pair* o, const float
     }
 #pragma unroll
     for (int f = 0; f < 6; f++) o[f] = n[f];
+}
+
+// cell_update() for a cell without sediment: o[3..5] (h2, q2x, q2y) are zero
+// pairs and acc[3..5] are zero.  The layer-2 sums stay zero, their damping
+// and the Coulomb scaling multiply zeros, and disImplicita can only take its
+// water-bottom case, so those steps are dropped; the layer-1 steps are
+// cell_update()'s, in its order.  o[3..5] are left as they are (zero).
+__device__ __forceinline__ void cell_update1(const KCell& k, Pair* o, const float* acc)
+{
+    const float eps = k.eps, eps4 = k.eps4, dt = k.dtf;
+    const float so1 = sdes(o[0].hi, eps, eps4);
+    const float uo1x = o[1].hi * so1, uo1y = o[2].hi * so1;
+
+    Pair n[3];
+#pragma unroll
+    for (int f = 0; f < 3; f++) n[f] = add(o[f], k.valf * acc[f]);
+
+    if (negative(n[0])) n[0] = Pair{0.0f, 0.0f};
+    {
+        const float lim = dt * 6.7108864e7f;
+        float z = n[0].hi * k.ieps;
+        float z4 = (z * z) * (z * z);
+        if (z4 < lim) {
+            float aux = fexp(-dt * rcp(z4 + LH_EPSF));
+            n[1] = scale(n[1], aux);
+            n[2] = scale(n[2], aux);
+        }
+    }
+    if (k.vmax1 > 0.0f) {
+        float h = n[0].hi, s = sdes(h, eps, eps4);
+        float ux = n[1].hi * s, uy = n[2].hi * s, u = fsqrt(ux * ux + uy * uy);
+        if (u > k.vmax1) { float c = k.vmax1 * rcp(u); n[1] = scale(n[1], c); n[2] = scale(n[2], c); }
+    }
+    {
+        const float h1 = n[0].hi;
+        const float s1 = sdes(h1, eps, eps4);
+        if (h1 > 0.0f) {
+            const float uo1 = fsqrt(uo1x * uo1x + uo1y * uo1y);
+            float c = dt * k.g * k.mf0 * k.mf0 * uo1 * rcp(__powf(h1, 4.0f / 3.0f) + LH_EPSF);
+            float f = h1 * s1 * rcp(1.0f + c);
+            n[1] = scale(n[1], f);
+            n[2] = scale(n[2], f);
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < 3; f++) o[f] = n[f];
 }
 
 }  // namespace f32

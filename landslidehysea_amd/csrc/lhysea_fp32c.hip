@@ -140,6 +140,25 @@ __global__ void k_finalize(Consts k, const float* __restrict__ Wn, const double*
     }
 }
 
+// Both cells of an edge without sediment: h2 exactly 0 (FP64) and q2 = 0.
+// When every edge of a wavefront is such an edge it takes edge1(), which
+// produces no layer-2 fluxes; the layer-2 accumulators then stay untouched.
+__device__ __forceinline__ bool no_sediment(const float* W0, const float* W1, double h20, double h21)
+{
+    return (h20 == 0.0) && (h21 == 0.0) && (W0[4] == 0.0f) && (W0[5] == 0.0f) && (W1[4] == 0.0f) &&
+           (W1[5] == 0.0f);
+}
+
+// sA[f][r][c] -= F[f] for the first NF fields (h1,q1x,q1y[,h2,q2x,q2y]), and
+// the wave-speed sum += lam
+template <int NF, int TY, int CW>
+__device__ __forceinline__ void accumulate(float (&sA)[7][TY][CW], int r, int c, const float* F, float lam)
+{
+#pragma unroll
+    for (int f = 0; f < NF; f++) sA[f][r][c] -= F[f];
+    sA[6][r][c] += lam;
+}
+
 // ----------------------------------------------------------------- fused step
 // One even-aligned TX x TY tile of one step (origin i0, j0): stage the tile
 // and its 1-cell halo, run the four edge classes in the reference's order,
@@ -234,20 +253,21 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
                 interior = true;
             }
             const bool has0 = (r0 >= 0 && r0 < TY), has1 = interior && (r0 + 1 < TY);
-            const float a0h1 = has0 ? sA[0][r0][c] : 0.0f, a0h2 = has0 ? sA[3][r0][c] : 0.0f;
-            if (has1) { a1h1 = sA[0][r0 + 1][c]; a1h2 = sA[3][r0 + 1][c]; }
+            const float a0h1 = has0 ? sA[0][r0][c] : 0.0f;
+            if (has1) a1h1 = sA[0][r0 + 1][c];
             EdgeFlux F;
-            if (edge(kc, W0, W1, h10, h20, h11, h21, H0, H1, 0.0f, n1y, kc.len_y, a0h1, a0h2, a1h1, a1h2,
-                     interior, F)) {
-                if (has0) {
-#pragma unroll
-                    for (int f = 0; f < 6; f++) sA[f][r0][c] -= F.Fm[f];
-                    sA[6][r0][c] += F.lam;
+            if (__all(no_sediment(W0, W1, h20, h21))) {
+                if (edge1(kc, W0, W1, h10, h11, H0, H1, 0.0f, n1y, kc.len_y, a0h1, a1h1, interior, F)) {
+                    if (has0) accumulate<3>(sA, r0, c, F.Fm, F.lam);
+                    if (has1) accumulate<3>(sA, r0 + 1, c, F.Fp, F.lam);
                 }
-                if (has1) {
-#pragma unroll
-                    for (int f = 0; f < 6; f++) sA[f][r0 + 1][c] -= F.Fp[f];
-                    sA[6][r0 + 1][c] += F.lam;
+            } else {
+                const float a0h2 = has0 ? sA[3][r0][c] : 0.0f;
+                if (has1) a1h2 = sA[3][r0 + 1][c];
+                if (edge(kc, W0, W1, h10, h20, h11, h21, H0, H1, 0.0f, n1y, kc.len_y, a0h1, a0h2, a1h1, a1h2,
+                         interior, F)) {
+                    if (has0) accumulate<6>(sA, r0, c, F.Fm, F.lam);
+                    if (has1) accumulate<6>(sA, r0 + 1, c, F.Fp, F.lam);
                 }
             }
         }
@@ -290,18 +310,21 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
                 n1x = 1.0f;
                 interior = true;
             }
-            const float a0h1 = sA[0][r][c0], a0h2 = sA[3][r][c0];
-            if (interior) { a1h1 = sA[0][r][c0 + 1]; a1h2 = sA[3][r][c0 + 1]; }
+            const float a0h1 = sA[0][r][c0];
+            if (interior) a1h1 = sA[0][r][c0 + 1];
             EdgeFlux F;
-            if (edge(kc, W0, W1, h10, h20, h11, h21, H0, H1, n1x, 0.0f, kc.len_x, a0h1, a0h2, a1h1, a1h2,
-                     interior, F)) {
-#pragma unroll
-                for (int f = 0; f < 6; f++) sA[f][r][c0] -= F.Fm[f];
-                sA[6][r][c0] += F.lam;
-                if (interior) {
-#pragma unroll
-                    for (int f = 0; f < 6; f++) sA[f][r][c0 + 1] -= F.Fp[f];
-                    sA[6][r][c0 + 1] += F.lam;
+            if (__all(no_sediment(W0, W1, h20, h21))) {
+                if (edge1(kc, W0, W1, h10, h11, H0, H1, n1x, 0.0f, kc.len_x, a0h1, a1h1, interior, F)) {
+                    accumulate<3>(sA, r, c0, F.Fm, F.lam);
+                    if (interior) accumulate<3>(sA, r, c0 + 1, F.Fp, F.lam);
+                }
+            } else {
+                const float a0h2 = sA[3][r][c0];
+                if (interior) a1h2 = sA[3][r][c0 + 1];
+                if (edge(kc, W0, W1, h10, h20, h11, h21, H0, H1, n1x, 0.0f, kc.len_x, a0h1, a0h2, a1h1, a1h2,
+                         interior, F)) {
+                    accumulate<6>(sA, r, c0, F.Fm, F.lam);
+                    if (interior) accumulate<6>(sA, r, c0 + 1, F.Fp, F.lam);
                 }
             }
         }
@@ -334,11 +357,8 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
         Pair o[6];
         const double h1d = sh1[r + 1][c + 1], h2d = sh2[r + 1][c + 1];
         o[0].hi = (float)h1d; o[0].lo = (float)(h1d - (double)o[0].hi);
-        o[3].hi = (float)h2d; o[3].lo = (float)(h2d - (double)o[3].hi);
         o[1] = Pair{sq[0][r + 1][c + 1], W[7 * N + g]};
         o[2] = Pair{sq[1][r + 1][c + 1], W[8 * N + g]};
-        o[4] = Pair{sq[2][r + 1][c + 1], W[10 * N + g]};
-        o[5] = Pair{sq[3][r + 1][c + 1], W[11 * N + g]};
         float ac[6];
 #pragma unroll
         for (int f = 0; f < 6; f++) ac[f] = sA[f][r][c + 1];
@@ -350,11 +370,29 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
                 emax[2 * g + 1] = tnow;
             }
         }
-        cell_update(kk, o, ac);
+        // no sediment and no layer-2 flux sums (a pair whose value is 0 has a
+        // zero compensation, so the q2 compensations need not be read)
+        const bool bare = (h2d == 0.0) && (sq[2][r + 1][c + 1] == 0.0f) && (sq[3][r + 1][c + 1] == 0.0f) &&
+                          (ac[3] == 0.0f) && (ac[4] == 0.0f) && (ac[5] == 0.0f);
+        if (__all(bare)) {
+            cell_update1(kk, o, ac);
 #pragma unroll
-        for (int f = 0; f < 6; f++) {
-            Wn[f * N + g] = o[f].hi;
-            Wn[(6 + f) * N + g] = o[f].lo;
+            for (int f = 0; f < 3; f++) {
+                Wn[f * N + g] = o[f].hi;
+                Wn[(6 + f) * N + g] = o[f].lo;
+                Wn[(3 + f) * N + g] = 0.0f;
+                Wn[(9 + f) * N + g] = 0.0f;
+            }
+        } else {
+            o[3].hi = (float)h2d; o[3].lo = (float)(h2d - (double)o[3].hi);
+            o[4] = Pair{sq[2][r + 1][c + 1], W[10 * N + g]};
+            o[5] = Pair{sq[3][r + 1][c + 1], W[11 * N + g]};
+            cell_update(kk, o, ac);
+#pragma unroll
+            for (int f = 0; f < 6; f++) {
+                Wn[f * N + g] = o[f].hi;
+                Wn[(6 + f) * N + g] = o[f].lo;
+            }
         }
     }
     return Amax;

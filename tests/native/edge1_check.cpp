@@ -1,0 +1,130 @@
+// CPU check of the FP32 no-sediment forms:
+//  * edge1() must give the same layer-1 fluxes and wave speed as edge() (equal
+//    values; zero results may differ in sign) and edge() zero layer-2 fluxes,
+//    on random edges whose two cells carry water only: wet, thin, dry and
+//    near-dry depths, steps in the bottom, interior and ghost (wall / open)
+//    edges in x and y;
+//  * cell_update1() must give cell_update()'s layer-1 state and cell_update()
+//    a zero layer-2 state, on random water-only cells with zero layer-2 sums.
+// Build: g++ -std=c++17 -O2 -ffp-contract=off -I tests/native edge1_check.cpp
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdio>
+#include <random>
+#include "../../landslidehysea_amd/csrc/lhysea_fp32.hpp"
+using namespace lhysea::f32;
+
+static bool same(float a, float b) { return a == b; }
+
+int main(int argc, char** argv)
+{
+    const long iters = argc > 1 ? std::atol(argv[1]) : 500000;
+    std::mt19937_64 rng(7);
+    std::uniform_real_distribution<double> U(0, 1);
+    long wet = 0, dry = 0, bad = 0;
+    for (long it = 0; it < iters; it++) {
+        K32 k{};
+        k.eps = 5e-3f;
+        k.eps4 = k.eps * k.eps * k.eps * k.eps;
+        k.g = 9.81f;
+        k.r = (float)(0.45 + 0.15 * U(rng));
+        k.gp = (float)(9.81 * (1.0 - k.r));
+        k.igp = (float)(1.0 / (9.81 * (1.0 - k.r)));
+        k.mu = 0.17f;
+        k.peso = 1.0f;
+        k.dtf = (float)(0.05 + 0.3 * U(rng));
+        k.area = 3091.0f;
+        k.len_x = k.len_y = 55.6f;
+        k.ten_eps = 10.0f / k.eps;
+        k.idt_eps = (float)(1.0 / ((double)k.dtf + (double)LH_EPSF));
+        k.bq = (float)(k.dtf / (double)k.area);
+        const double H0 = 400 * U(rng);
+        double H1 = std::max(0.0, H0 + (U(rng) < 0.3 ? 0.0 : 20 * (U(rng) - 0.5)));
+        auto depth = [&](double H) {
+            const double r = U(rng);
+            return r < 0.15 ? 0.0 : r < 0.3 ? 0.01 * U(rng) : r < 0.4 ? 1e-4 * U(rng) : H * U(rng);
+        };
+        double h10 = depth(H0), h11 = depth(H1);
+        float W0[6] = {(float)h10, 0, 0, 0, 0, 0}, W1[6] = {(float)h11, 0, 0, 0, 0, 0};
+        for (int f = 1; f < 3; f++) {
+            W0[f] = (float)((U(rng) - 0.5) * 20 * (h10 > 0));
+            W1[f] = (float)((U(rng) - 0.5) * 20 * (h11 > 0));
+        }
+        const bool interior = U(rng) < 0.85;
+        float n1x = 1, n1y = 0, len = k.len_x;
+        if (U(rng) < 0.5) {
+            n1x = 0;
+            n1y = 1;
+            len = k.len_y;
+        }
+        if (!interior) {
+            const float bd = U(rng) < 0.5 ? -1.0f : 1.0f;
+            for (int f = 0; f < 6; f++) W1[f] = W0[f];
+            if (n1x != 0) W1[1] *= bd;
+            else W1[2] *= bd;
+            h11 = h10;
+            H1 = H0;
+            if (U(rng) < 0.5) {
+                n1x = -n1x;
+                n1y = -n1y;
+            }
+        }
+        const float a0 = (float)((U(rng) - 0.5) * 100), a1 = (float)((U(rng) - 0.5) * 100);
+        EdgeFlux A, B;
+        const bool wa = edge(k, W0, W1, h10, 0.0, h11, 0.0, H0, H1, n1x, n1y, len, a0, 0.0f, a1, 0.0f, interior, A);
+        const bool wb = edge1(k, W0, W1, h10, h11, H0, H1, n1x, n1y, len, a0, a1, interior, B);
+        bool ok = wa == wb;
+        if (ok && wa) {
+            ok = same(A.lam, B.lam);
+            for (int f = 0; f < 3; f++) ok = ok && same(A.Fm[f], B.Fm[f]) && same(A.Fp[f], B.Fp[f]);
+            for (int f = 3; f < 6; f++) ok = ok && A.Fm[f] == 0.0f && A.Fp[f] == 0.0f;
+        }
+        wet += wa;
+        dry += !wa;
+        if (!ok && ++bad <= 5)
+            std::printf("mismatch at %ld: Fm %g %g %g / %g %g %g, lam %g / %g\n", it, A.Fm[0], A.Fm[1], A.Fm[2],
+                        B.Fm[0], B.Fm[1], B.Fm[2], A.lam, B.lam);
+    }
+    std::printf("edges %ld wet %ld dry %ld mismatches %ld\n", iters, wet, dry, bad);
+
+    long cbad = 0;
+    for (long it = 0; it < iters; it++) {
+        KCell k{};
+        k.eps = 5e-3f;
+        k.eps4 = k.eps * k.eps * k.eps * k.eps;
+        k.r = (float)(0.45 + 0.15 * U(rng));
+        k.g = 9.81f;
+        k.mu = 0.17f;
+        k.mfc = 0.08f;
+        k.mf0 = (float)(0.03 * U(rng));
+        k.mfs = 0.03f;
+        k.vmax1 = U(rng) < 0.3 ? (float)(5 * U(rng)) : 0.0f;
+        k.vmax2 = U(rng) < 0.3 ? (float)(5 * U(rng)) : 0.0f;
+        k.dtf = (float)(0.05 + 0.3 * U(rng));
+        k.valf = (float)(k.dtf / 3091.0);
+        k.ieps = 1.0f / k.eps;
+        k.ten_eps = 10.0f / k.eps;
+        const double r = U(rng);
+        const double h = r < 0.1 ? 0.0 : r < 0.3 ? 0.02 * U(rng) : 300 * U(rng);
+        Pair o[6], p[6];
+        o[0] = Pair{(float)h, (float)(h - (double)(float)h)};
+        for (int f = 1; f < 3; f++) {
+            const double q = (U(rng) - 0.5) * 40 * (h > 0);
+            o[f] = Pair{(float)q, (float)(q - (double)(float)q)};
+        }
+        for (int f = 3; f < 6; f++) o[f] = Pair{0.0f, 0.0f};
+        float acc[6] = {0, 0, 0, 0, 0, 0};
+        for (int f = 0; f < 3; f++) acc[f] = (float)((U(rng) - 0.5) * (f == 0 ? 2e3 : 2e4));
+        for (int f = 0; f < 6; f++) p[f] = o[f];
+        cell_update(k, o, acc);
+        cell_update1(k, p, acc);
+        bool ok = true;
+        for (int f = 0; f < 3; f++) ok = ok && same(o[f].hi, p[f].hi) && same(o[f].lo, p[f].lo);
+        for (int f = 3; f < 6; f++) ok = ok && o[f].hi == 0.0f && o[f].lo == 0.0f && p[f].hi == 0.0f && p[f].lo == 0.0f;
+        if (!ok && ++cbad <= 5)
+            std::printf("cell mismatch at %ld: %g %g %g / %g %g %g\n", it, o[0].hi, o[1].hi, o[2].hi, p[0].hi, p[1].hi,
+                        p[2].hi);
+    }
+    std::printf("cells %ld mismatches %ld\n", iters, cbad);
+    return (bad + cbad) != 0;
+}
