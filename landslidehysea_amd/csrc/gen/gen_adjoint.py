@@ -272,6 +272,16 @@ def build_cell():
     return P
 
 
+# inputs that are exactly zero at a sediment-free edge / cell, and the signs
+# the parameters always have (eps_h > 0, g > 0, 0 < r < 1, dt > 0, ...)
+EDGE_ZEROS = ("h20", "h21", "q02", "q03", "q12", "q13", "a0h2", "a1h2")
+EDGE_POS = ("eps", "eps4", "g", "gp", "r", "dt", "area", "len")
+EDGE_NONNEG = ("mu", "h10", "h11")
+CELL_ZEROS = ("o3", "o4", "o5", "a3", "a4", "a5")
+CELL_POS = ("eps", "eps4", "r", "g", "dt", "val")
+CELL_NONNEG = ("mfc", "mf0", "mfs", "vmax1", "vmax2", "o0")
+
+
 def main():
     text = ["// Generated by landslidehysea_amd/csrc/gen/gen_adjoint.py -- do not edit.",
             "// Forward recomputation + reverse sweep of the edge and the cell update",
@@ -279,6 +289,14 @@ def main():
     text.append(emit(build_edge(), forward_only=True))
     text.append(emit(build_edge()))
     text.append(emit(build_cell()))
+    # the same programs at states without sediment: every layer-2 input is 0
+    # (the adjoints of those inputs are still produced)
+    text.append("// Specialisations for edges / cells without sediment (h2 = q2 = 0, zero layer-2")
+    text.append("// partial and flux sums): forward terms and reverse contributions that these")
+    text.append("// zeros remove are dropped; results equal the general functions' up to the")
+    text.append("// sign of zero results.")
+    text.append(emit(build_edge(), name="adj_edge_ns", zeros=EDGE_ZEROS, positive=EDGE_POS, nonneg=EDGE_NONNEG))
+    text.append(emit(build_cell(), name="adj_cell_ns", zeros=CELL_ZEROS, positive=CELL_POS, nonneg=CELL_NONNEG))
     with open(OUT, "w") as f:
         f.write("\n".join(text))
     print(OUT)

@@ -213,10 +213,20 @@ __global__ void __launch_bounds__(256) k_cell_adj(Consts k, SV<F> sv, Work<F> w,
             a[f] = w.ACC[f * N + c];
             gn[f] = (F)w.LAM[f * N + c];
         }
-        adj_cell<F>(eps, eps * eps * eps * eps, (F)k.r, (F)k.g, (F)k.mfc, (F)k.mf0, (F)k.mfs, (F)dt,
-                    (F)(dt / k.area), (F)k.vmax1, (F)k.vmax2, o[0], o[1], o[2], o[3], o[4], o[5], a[0], a[1], a[2],
-                    a[3], a[4], a[5], (F)k.mu, n[0], n[1], n[2], n[3], n[4], n[5], gn[0], gn[1], gn[2], gn[3], gn[4],
-                    gn[5], go[0], go[1], go[2], go[3], go[4], go[5], ga[0], ga[1], ga[2], ga[3], ga[4], ga[5], gmu);
+#define LH_CELL_ARGS                                                                                           \
+    eps, eps * eps * eps * eps, (F)k.r, (F)k.g, (F)k.mfc, (F)k.mf0, (F)k.mfs, (F)dt, (F)(dt / k.area), (F)k.vmax1, \
+        (F)k.vmax2, o[0], o[1], o[2], o[3], o[4], o[5], a[0], a[1], a[2], a[3], a[4], a[5], (F)k.mu, n[0], n[1],   \
+        n[2], n[3], n[4], n[5], gn[0], gn[1], gn[2], gn[3], gn[4], gn[5], go[0], go[1], go[2], go[3], go[4], go[5], \
+        ga[0], ga[1], ga[2], ga[3], ga[4], ga[5], gmu
+        // a wavefront of cells without sediment and layer-2 sums takes the
+        // specialisation (same values and adjoints)
+        const bool bare = (o[3] == F(0)) && (o[4] == F(0)) && (o[5] == F(0)) && (a[3] == F(0)) && (a[4] == F(0)) &&
+                          (a[5] == F(0));
+        if (__all(bare))
+            adj_cell_ns<F>(LH_CELL_ARGS);
+        else
+            adj_cell<F>(LH_CELL_ARGS);
+#undef LH_CELL_ARGS
         for (int f = 0; f < 6; f++) {
             w.LAM[f * N + c] = (double)go[f];
             w.AB[f * N + c] = ga[f];
@@ -256,12 +266,22 @@ __global__ void __launch_bounds__(256) k_edge_adj(Consts k, SV<F> sv, const doub
     }
     F fm[6], fp[6], gq0[4], gq1[4], ga0h1, ga0h2, ga1h1, ga1h2;
     double gh10, gh20, gh11, gh21;
-    adj_edge<F>((F)c.eps, (F)c.eps4, (F)c.g, (F)c.gp, (F)c.r, (F)c.mu, (F)c.dt, (F)c.area, (F)r.len, (F)r.n1x,
-                (F)r.n1y, r.interior ? F(1) : F(0), in.H0, in.H1, in.h10, in.h20, in.h11, in.h21, in.q0[0], in.q0[1],
-                in.q0[2], in.q0[3], in.q1[0], in.q1[1], in.q1[2], in.q1[3], a0h1, a0h2, a1h1, a1h2, fm[0], fm[1],
-                fm[2], fm[3], fm[4], fm[5], fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], gfm[0], gfm[1], gfm[2],
-                gfm[3], gfm[4], gfm[5], gfp[0], gfp[1], gfp[2], gfp[3], gfp[4], gfp[5], gh10, gh20, gh11, gh21,
-                gq0[0], gq0[1], gq0[2], gq0[3], gq1[0], gq1[1], gq1[2], gq1[3], ga0h1, ga0h2, ga1h1, ga1h2);
+#define LH_EDGE_ARGS                                                                                            \
+    (F)c.eps, (F)c.eps4, (F)c.g, (F)c.gp, (F)c.r, (F)c.mu, (F)c.dt, (F)c.area, (F)r.len, (F)r.n1x, (F)r.n1y,        \
+        r.interior ? F(1) : F(0), in.H0, in.H1, in.h10, in.h20, in.h11, in.h21, in.q0[0], in.q0[1], in.q0[2],        \
+        in.q0[3], in.q1[0], in.q1[1], in.q1[2], in.q1[3], a0h1, a0h2, a1h1, a1h2, fm[0], fm[1], fm[2], fm[3], fm[4], \
+        fm[5], fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], gfm[0], gfm[1], gfm[2], gfm[3], gfm[4], gfm[5], gfp[0],     \
+        gfp[1], gfp[2], gfp[3], gfp[4], gfp[5], gh10, gh20, gh11, gh21, gq0[0], gq0[1], gq0[2], gq0[3], gq1[0],      \
+        gq1[1], gq1[2], gq1[3], ga0h1, ga0h2, ga1h1, ga1h2
+    // a wavefront of edges whose cells carry no sediment (and no layer-2
+    // partial sums) takes the specialisation (same values and adjoints)
+    const bool bare = (in.h20 == 0.0) && (in.h21 == 0.0) && (in.q0[2] == F(0)) && (in.q0[3] == F(0)) &&
+                      (in.q1[2] == F(0)) && (in.q1[3] == F(0)) && (a0h2 == F(0)) && (a1h2 == F(0));
+    if (__all(bare))
+        adj_edge_ns<F>(LH_EDGE_ARGS);
+    else
+        adj_edge<F>(LH_EDGE_ARGS);
+#undef LH_EDGE_ARGS
     double* L = w.LAM;
     if (r.interior) {
         L[r.c0] += gh10;
@@ -317,6 +337,265 @@ __global__ void k_inject(double* LAM, size_t N, const int* gauges, int ng, const
 __global__ void k_set_step(Ctl* ctl, int step)
 {
     if (threadIdx.x == 0) ctl->step = step;
+}
+
+// ------------------------------------------------------- fused reverse step
+// One workgroup computes lambda^n of a TX x TY tile (even origin) in one pass,
+// FP32 adjoint.  lambda^n of a cell depends on the cell adjoints and on the
+// edge adjoints of a cone that reaches exactly two cells beyond the tile on
+// every side (each class-k edge needs its cells' flux-sum adjoints after the
+// later classes' partial-sum adjoints, which come from the later-class edges
+// of those cells), so the workgroup stages the (TX+4) x (TY+4) box from HBM:
+// state, depth, the partial sums P and flux sums ACC of the recomputed
+// forward step, and lambda^{n+1}.  It then runs the cell adjoint of every box
+// cell (flux-sum adjoints in LDS; the direct part of lambda^n and d/dmu for the
+// tile's own cells), the four edge classes in reverse order over every edge
+// inside the box (LDS updates; within a class no two edges share a cell), and
+// writes lambda^n of its own cells.  lambda^{n+1} is read from one buffer and
+// lambda^n written to another.
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_adj_fused(Consts k, const float* __restrict__ W,
+                                                  const double* __restrict__ H, const float* __restrict__ P,
+                                                  const float* __restrict__ ACC, const double* __restrict__ LIN,
+                                                  double* __restrict__ LOUT, double* gmu,
+                                                  const double* __restrict__ dt_hist, int step, int tiles_x,
+                                                  int tiles_y)
+{
+    static_assert(TX % 2 == 0 && TY % 2 == 0, "tile origins must stay even");
+    constexpr int HW = 2, CW = TX + 2 * HW, RW = TY + 2 * HW;
+    __shared__ double sh1[RW][CW], sh2[RW][CW], sH[RW][CW];
+    __shared__ float sq[4][RW][CW];
+    __shared__ float sP[3][2][RW][CW];
+    __shared__ float sAB[6][RW][CW];
+    __shared__ double sL[6][TY][TX];
+    __shared__ double red[NT / 64];
+
+    const int nx = k.nx, ny = k.ny;
+    const size_t N = (size_t)nx * ny;
+    const int nb = tiles_x * tiles_y;
+    int b;
+    {
+        const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+        const int q = nb >> 3, rem = nb & 7;
+        b = xcd * q + (xcd < rem ? xcd : rem) + slot;
+    }
+    const int i0 = (b % tiles_x) * TX, j0 = (b / tiles_x) * TY;
+    const int bi = i0 - HW, bj = j0 - HW;          // global index of box cell (0, 0)
+    const int tid = threadIdx.x;
+    const double dt = dt_hist[step];
+    const EC c = edge_consts(k, dt);
+    const float eps = (float)k.eps_h;
+
+    // 1. stage the box; cell adjoint of every box cell
+    double gm = 0.0;
+    for (int e = tid; e < RW * CW; e += NT) {
+        const int r = e / CW, cc = e % CW;
+        const int gj = bj + r, gi = bi + cc;
+        if (gj < 0 || gj >= ny || gi < 0 || gi >= nx) {
+#pragma unroll
+            for (int f = 0; f < 6; f++) sAB[f][r][cc] = 0.0f;
+            continue;
+        }
+        const size_t g = (size_t)gj * nx + gi;
+        const double h1 = (double)W[g] + (double)W[6 * N + g], h2 = (double)W[3 * N + g] + (double)W[9 * N + g];
+        float o[6] = {(float)h1, W[N + g], W[2 * N + g], (float)h2, W[4 * N + g], W[5 * N + g]};
+        sh1[r][cc] = h1;
+        sh2[r][cc] = h2;
+        sH[r][cc] = H[g];
+        sq[0][r][cc] = o[1];
+        sq[1][r][cc] = o[2];
+        sq[2][r][cc] = o[4];
+        sq[3][r][cc] = o[5];
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            sP[m][0][r][cc] = P[(size_t)m * 2 * N + g];
+            sP[m][1][r][cc] = P[(size_t)m * 2 * N + N + g];
+        }
+        float a[6], gn[6], n[6], go[6], ga[6], gmuc;
+#pragma unroll
+        for (int f = 0; f < 6; f++) {
+            a[f] = ACC[f * N + g];
+            gn[f] = (float)LIN[f * N + g];
+        }
+#define LH_CELL_ARGS                                                                                               \
+    eps, eps * eps * eps * eps, (float)k.r, (float)k.g, (float)k.mfc, (float)k.mf0, (float)k.mfs, (float)dt,           \
+        (float)(dt / k.area), (float)k.vmax1, (float)k.vmax2, o[0], o[1], o[2], o[3], o[4], o[5], a[0], a[1], a[2],    \
+        a[3], a[4], a[5], (float)k.mu, n[0], n[1], n[2], n[3], n[4], n[5], gn[0], gn[1], gn[2], gn[3], gn[4], gn[5],  \
+        go[0], go[1], go[2], go[3], go[4], go[5], ga[0], ga[1], ga[2], ga[3], ga[4], ga[5], gmuc
+        const bool bare = (o[3] == 0.0f) && (o[4] == 0.0f) && (o[5] == 0.0f) && (a[3] == 0.0f) && (a[4] == 0.0f) &&
+                          (a[5] == 0.0f);
+        if (__all(bare))
+            adj_cell_ns<float>(LH_CELL_ARGS);
+        else
+            adj_cell<float>(LH_CELL_ARGS);
+#undef LH_CELL_ARGS
+#pragma unroll
+        for (int f = 0; f < 6; f++) sAB[f][r][cc] = ga[f];
+        const int tr = r - HW, tc = cc - HW;
+        if (tr >= 0 && tr < TY && tc >= 0 && tc < TX) {
+#pragma unroll
+            for (int f = 0; f < 6; f++) sL[f][tr][tc] = (double)go[f];
+            gm += (double)gmuc;
+        }
+    }
+    __syncthreads();
+
+    // 2. edge adjoints, classes 3, 2, 1, 0.  A class-cls edge joins box cells
+    //    (r0, c0) and (r1, c1), or is the ghost edge of (r0, c0) on the domain
+    //    boundary.
+    auto edge_adj = [&](int cls, int r0, int c0, int r1, int c1, bool interior, float n1x, float n1y, float len,
+                        float bd) {
+        const int rr1 = interior ? r1 : r0, cc1 = interior ? c1 : c0;
+        EdgeIn<float> in;
+        in.h10 = sh1[r0][c0];
+        in.h20 = sh2[r0][c0];
+        in.h11 = sh1[rr1][cc1];
+        in.h21 = sh2[rr1][cc1];
+        in.H0 = sH[r0][c0];
+        in.H1 = sH[rr1][cc1];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            in.q0[q] = sq[q][r0][c0];
+            in.q1[q] = sq[q][rr1][cc1];
+        }
+        const bool xedge = cls >= 2;
+        if (!interior) {
+            const int nk = xedge ? 0 : 1;
+            in.q1[nk] *= bd;
+            in.q1[2 + nk] *= bd;
+        }
+        float a0h1 = 0.0f, a0h2 = 0.0f, a1h1 = 0.0f, a1h2 = 0.0f;
+        if (cls > 0) {
+            a0h1 = sP[cls - 1][0][r0][c0];
+            a0h2 = sP[cls - 1][1][r0][c0];
+            if (interior) {
+                a1h1 = sP[cls - 1][0][r1][c1];
+                a1h2 = sP[cls - 1][1][r1][c1];
+            }
+        }
+        float gfm[6], gfp[6];
+#pragma unroll
+        for (int f = 0; f < 6; f++) {
+            gfm[f] = -sAB[f][r0][c0];
+            gfp[f] = interior ? -sAB[f][r1][c1] : 0.0f;
+        }
+        float fm[6], fp[6], gq0[4], gq1[4], ga0h1, ga0h2, ga1h1, ga1h2;
+        double gh10, gh20, gh11, gh21;
+#define LH_EDGE_ARGS                                                                                                \
+    c.eps, c.eps4, c.g, c.gp, c.r, c.mu, c.dt, c.area, len, n1x, n1y, interior ? 1.0f : 0.0f, in.H0, in.H1, in.h10,   \
+        in.h20, in.h11, in.h21, in.q0[0], in.q0[1], in.q0[2], in.q0[3], in.q1[0], in.q1[1], in.q1[2], in.q1[3], a0h1,  \
+        a0h2, a1h1, a1h2, fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], gfm[0], \
+        gfm[1], gfm[2], gfm[3], gfm[4], gfm[5], gfp[0], gfp[1], gfp[2], gfp[3], gfp[4], gfp[5], gh10, gh20, gh11,     \
+        gh21, gq0[0], gq0[1], gq0[2], gq0[3], gq1[0], gq1[1], gq1[2], gq1[3], ga0h1, ga0h2, ga1h1, ga1h2
+        const bool bare = (in.h20 == 0.0) && (in.h21 == 0.0) && (in.q0[2] == 0.0f) && (in.q0[3] == 0.0f) &&
+                          (in.q1[2] == 0.0f) && (in.q1[3] == 0.0f) && (a0h2 == 0.0f) && (a1h2 == 0.0f);
+        if (__all(bare))
+            adj_edge_ns<float>(LH_EDGE_ARGS);
+        else
+            adj_edge<float>(LH_EDGE_ARGS);
+#undef LH_EDGE_ARGS
+        // lambda^n of the tile's own cells
+        auto add_lambda = [&](int r, int cc, double dh1, double dh2, const double* dq) {
+            const int tr = r - HW, tc = cc - HW;
+            if (tr < 0 || tr >= TY || tc < 0 || tc >= TX) return;
+            sL[0][tr][tc] += dh1;
+            sL[3][tr][tc] += dh2;
+            sL[1][tr][tc] += dq[0];
+            sL[2][tr][tc] += dq[1];
+            sL[4][tr][tc] += dq[2];
+            sL[5][tr][tc] += dq[3];
+        };
+        if (interior) {
+            const double d0[4] = {(double)gq0[0], (double)gq0[1], (double)gq0[2], (double)gq0[3]};
+            const double d1[4] = {(double)gq1[0], (double)gq1[1], (double)gq1[2], (double)gq1[3]};
+            add_lambda(r0, c0, gh10, gh20, d0);
+            add_lambda(r1, c1, gh11, gh21, d1);
+            sAB[0][r1][c1] += ga1h1;
+            sAB[3][r1][c1] += ga1h2;
+        } else {
+            // the ghost copies cell 0 with its normal discharge times borde
+            const int nk = xedge ? 0 : 1;
+            double d0[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) d0[q] = (double)gq0[q] + (double)gq1[q] * (((q & 1) == nk) ? (double)bd : 1.0);
+            add_lambda(r0, c0, gh10 + gh11, gh20 + gh21, d0);
+        }
+        sAB[0][r0][c0] += ga0h1;
+        sAB[3][r0][c0] += ga0h2;
+    };
+
+#pragma unroll 1
+    for (int cls = 3; cls >= 0; cls--) {
+        if (cls >= 2) {
+            // x-edges: box edge column cl joins box columns cl-1 and cl (global ie = bi + cl)
+            const int par = cls - 2;
+            const int first = ((bi + 1) & 1) == par ? 1 : 2;       // first cl >= 1 with (bi + cl) % 2 == par
+            const int ncol = (CW - 1 - first) / 2 + 1;             // cl = first, first + 2, ... <= CW - 1
+            for (int e = tid; e < ncol * RW; e += NT) {
+                const int r = e / ncol, cl = first + 2 * (e % ncol);
+                const int gj = bj + r, ie = bi + cl;
+                if (gj < 0 || gj >= ny || ie < 0 || ie > nx) continue;
+                if (ie == 0)
+                    edge_adj(cls, r, cl, r, cl, false, -1.0f, 0.0f, (float)k.dy, (float)k.borde_izq);
+                else if (ie == nx)
+                    edge_adj(cls, r, cl - 1, r, cl - 1, false, 1.0f, 0.0f, (float)k.dy, (float)k.borde_der);
+                else
+                    edge_adj(cls, r, cl - 1, r, cl, true, 1.0f, 0.0f, (float)k.dy, 1.0f);
+            }
+            // ghost edges on the box's outer edge columns (cl = 0: ie = 0; cl = CW: ie = nx)
+            const bool gl = (bi == 0) && (par == 0), gr = (bi + CW == nx) && ((nx & 1) == par);
+            if (gl || gr)
+                for (int r = tid; r < RW; r += NT) {
+                    const int gj = bj + r;
+                    if (gj < 0 || gj >= ny) continue;
+                    if (gl) edge_adj(cls, r, 0, r, 0, false, -1.0f, 0.0f, (float)k.dy, (float)k.borde_izq);
+                    if (gr) edge_adj(cls, r, CW - 1, r, CW - 1, false, 1.0f, 0.0f, (float)k.dy, (float)k.borde_der);
+                }
+        } else {
+            // y-edges: box edge row rl joins box rows rl-1 and rl (global je = bj + rl)
+            const int first = ((bj + 1) & 1) == cls ? 1 : 2;
+            const int nrow = (RW - 1 - first) / 2 + 1;
+            for (int e = tid; e < nrow * CW; e += NT) {
+                const int cc = e % CW, rl = first + 2 * (e / CW);
+                const int gi = bi + cc, je = bj + rl;
+                if (gi < 0 || gi >= nx || je < 0 || je > ny) continue;
+                if (je == 0)
+                    edge_adj(cls, rl, cc, rl, cc, false, 0.0f, -1.0f, (float)k.dx, (float)k.borde_sup);
+                else if (je == ny)
+                    edge_adj(cls, rl - 1, cc, rl - 1, cc, false, 0.0f, 1.0f, (float)k.dx, (float)k.borde_inf);
+                else
+                    edge_adj(cls, rl - 1, cc, rl, cc, true, 0.0f, 1.0f, (float)k.dx, 1.0f);
+            }
+            // ghost edges on the box's outer edge rows (rl = 0: je = 0; rl = RW: je = ny)
+            const bool gt = (bj == 0) && (cls == 0), gb = (bj + RW == ny) && ((ny & 1) == cls);
+            if (gt || gb)
+                for (int cc = tid; cc < CW; cc += NT) {
+                    const int gi = bi + cc;
+                    if (gi < 0 || gi >= nx) continue;
+                    if (gt) edge_adj(cls, 0, cc, 0, cc, false, 0.0f, -1.0f, (float)k.dx, (float)k.borde_sup);
+                    if (gb) edge_adj(cls, RW - 1, cc, RW - 1, cc, false, 0.0f, 1.0f, (float)k.dx, (float)k.borde_inf);
+                }
+        }
+        __syncthreads();
+    }
+
+    // 3. lambda^n of the tile's own cells, d/dmu
+    for (int e = tid; e < TX * TY; e += NT) {
+        const int tr = e / TX, tc = e % TX;
+        const int gj = j0 + tr, gi = i0 + tc;
+        if (gj >= ny || gi >= nx) continue;
+        const size_t g = (size_t)gj * nx + gi;
+#pragma unroll
+        for (int f = 0; f < 6; f++) LOUT[f * N + g] = sL[f][tr][tc];
+    }
+    for (int off = 32; off > 0; off >>= 1) gm += __shfl_xor(gm, off, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = gm;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int w = 0; w < NT / 64; w++) s += red[w];
+        if (s != 0.0) atomicAdd(gmu, s);
+    }
 }
 
 }  // namespace adj
@@ -416,13 +695,19 @@ hipError_t reverse_step_t(const Consts& k, const F* W, const double* H, F* P, F*
     const size_t N = (size_t)k.nx * k.ny;
     adj::SV<F> sv{W, N};
     adj::Work<F> w{P, ACC, AB, LAM, gmu};
-    hipMemsetAsync(ACC, 0, 6 * N * sizeof(F), s);
     long ne[4];
     for (int cls = 0; cls < 4; cls++)
         ne[cls] = (cls < 2) ? (long)((k.ny - cls) / 2 + 1) * k.nx : (long)((k.nx - (cls - 2)) / 2 + 1) * k.ny;
-    for (int cls = 0; cls < 4; cls++)
-        hipLaunchKernelGGL(adj::k_recompute<F>, dim3((unsigned)((ne[cls] + 255) / 256)), dim3(256), 0, s, k, sv, H,
-                           w, dt_hist, step, cls);
+    if constexpr (sizeof(F) == 4) {
+        // the forward step's own tile kernel recomputes the partial and final sums
+        hipError_t e = f32c_recompute(k, (const float*)W, H, dt_hist, step, (float*)P, (float*)ACC, s);
+        if (e != hipSuccess) return e;
+    } else {
+        hipMemsetAsync(ACC, 0, 6 * N * sizeof(F), s);
+        for (int cls = 0; cls < 4; cls++)
+            hipLaunchKernelGGL(adj::k_recompute<F>, dim3((unsigned)((ne[cls] + 255) / 256)), dim3(256), 0, s, k, sv,
+                               H, w, dt_hist, step, cls);
+    }
     hipLaunchKernelGGL(adj::k_cell_adj<F>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, k, sv, w, dt_hist,
                        step);
     for (int cls = 3; cls >= 0; cls--)
@@ -435,6 +720,19 @@ hipError_t adj_reverse_step_f32(const Consts& k, const float* W, const double* H
                                 double* LAM, double* gmu, const double* dt_hist, int step, hipStream_t s)
 {
     return reverse_step_t<float>(k, W, H, P, ACC, AB, LAM, gmu, dt_hist, step, s);
+}
+
+hipError_t adj_reverse_step_f32_fused(const Consts& k, const float* W, const double* H, float* P, float* ACC,
+                                      const double* LAMin, double* LAMout, double* gmu, const double* dt_hist,
+                                      int step, hipStream_t s)
+{
+    constexpr int TX = 60, TY = 12, NT = 512;
+    hipError_t e = f32c_recompute(k, W, H, dt_hist, step, P, ACC, s);
+    if (e != hipSuccess) return e;
+    const int tiles_x = (k.nx + TX - 1) / TX, tiles_y = (k.ny + TY - 1) / TY;
+    hipLaunchKernelGGL((adj::k_adj_fused<TX, TY, NT>), dim3((unsigned)(tiles_x * tiles_y)), dim3(NT), 0, s, k, W, H,
+                       P, ACC, LAMin, LAMout, gmu, dt_hist, step, tiles_x, tiles_y);
+    return hipGetLastError();
 }
 
 hipError_t adj_reverse_step_f64(const Consts& k, const double* W, const double* H, double* P, double* ACC, double* AB,

@@ -471,7 +471,8 @@ struct AdjCtx {
     void* P;               // partial sums [3][2][N]
     void* ACC;             // [6][N]
     void* AB;              // [6][N]
-    double* LAM;           // [6][N]
+    double* LAM;           // [6][N] state adjoint lambda of the current step
+    double* LAM2;          // [6][N] second lambda buffer of the fused FP32 reverse step
     double* gmu;
     int* gauges;
     double* gser;          // gauge series of the forward pass
@@ -511,9 +512,12 @@ hipError_t hook_reverse(void* c, int k)
     if (s->p.precision == LHYSEA_FP64)
         e = adj_reverse_step_f64(s->k, s->b64.W[s->cur], s->b64.H, (double*)a->P, (double*)a->ACC, (double*)a->AB,
                                  a->LAM, a->gmu, s->dt_hist, k, s->stream);
-    else
-        e = adj_reverse_step_f32(s->k, s->b32.W[s->cur], s->b32.H, (float*)a->P, (float*)a->ACC, (float*)a->AB,
-                                 a->LAM, a->gmu, s->dt_hist, k, s->stream);
+    else {
+        // lambda^{n+1} in LAM, lambda^n into LAM2, then swap
+        e = adj_reverse_step_f32_fused(s->k, s->b32.W[s->cur], s->b32.H, (float*)a->P, (float*)a->ACC, a->LAM,
+                                       a->LAM2, a->gmu, s->dt_hist, k, s->stream);
+        std::swap(a->LAM, a->LAM2);
+    }
     if (e != hipSuccess) return e;
     if (k >= 1) e = adj_inject(a->LAM, s->N, a->gauges, a->ng, a->G + (size_t)(k - 1) * a->ng, s->stream);
     return e;
@@ -549,6 +553,7 @@ int lhysea_adjoint(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const int32_t* 
         hipFree(a.ACC);
         hipFree(a.AB);
         hipFree(a.LAM);
+        hipFree(a.LAM2);
         hipFree(a.gmu);
         hipFree(a.G);
         if (e0) hipEventDestroy(e0);
@@ -568,7 +573,9 @@ int lhysea_adjoint(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const int32_t* 
     };
     if (hipMalloc(&a.snap, (size_t)nsnap * a.state_bytes) != hipSuccess || hipMalloc(&a.P, 6 * N * fsz) != hipSuccess ||
         hipMalloc(&a.ACC, 6 * N * fsz) != hipSuccess || hipMalloc(&a.AB, 6 * N * fsz) != hipSuccess ||
-        hipMalloc(&a.LAM, 6 * N * sizeof(double)) != hipSuccess || hipMalloc(&a.gmu, sizeof(double)) != hipSuccess ||
+        hipMalloc(&a.LAM, 6 * N * sizeof(double)) != hipSuccess ||
+        (!f64 && hipMalloc(&a.LAM2, 6 * N * sizeof(double)) != hipSuccess) ||
+        hipMalloc(&a.gmu, sizeof(double)) != hipSuccess ||
         hipMalloc(&a.G, (size_t)nsteps * (ng > 0 ? ng : 1) * sizeof(double)) != hipSuccess)
         return done(fail(LHYSEA_ERR_GPU_MEMORY, "adjoint workspace allocation failed"));
     hipEventCreate(&e0);

@@ -168,10 +168,14 @@ __device__ __forceinline__ void accumulate(float (&sA)[7][TY][CW], int r, int c,
 // sA (FP32 accumulators: dh1,dq1x,dq1y,dh2,dq2x,dq2y,A).
 // Every cell receives its class-C1..C4 contributions in that order whatever
 // the tile shape, so the result does not depend on TX, TY or NT.
-template <int TX, int TY, int NT>
+// RECOMP: instead of updating the state, write the partial h1/h2 sums that
+// classes 2..4 read (P, [3][2][N]) and the final flux sums (ACC, [6][N]) of
+// the tile's cells: the forward recomputation of the discrete adjoint.
+template <int TX, int TY, int NT, bool RECOMP = false>
 __device__ __forceinline__ double tile_step(const Consts& k, const float* __restrict__ W, float* __restrict__ Wn,
-                                            const double* __restrict__ H, double dt, const Ctl* ctl,
-                                            double* __restrict__ emax, int i0, int j0)
+                                            const double* __restrict__ H, double dt, double tnow,
+                                            double* __restrict__ emax, int i0, int j0,
+                                            float* __restrict__ P = nullptr, float* __restrict__ ACC = nullptr)
 {
     static_assert(TX % 2 == 0 && TY % 2 == 0, "tile origins must stay even");
     constexpr int CW = TX + 2, RW = TY + 2;
@@ -206,6 +210,20 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
         for (int f = 0; f < 7; f++) sA[f][r][c] = 0.0f;
     }
     __syncthreads();
+
+    // RECOMP: the tile's own h1/h2 partial sums after class cls (read by class cls + 1)
+    auto store_partials = [&](int cls) {
+        float* Pc = P + (size_t)cls * 2 * N;
+        for (int e = tid; e < TX * TY; e += NT) {
+            const int r = e / TX, c = e % TX;
+            const int gj = j0 + r, gi = i0 + c;
+            if (gj >= ny || gi >= nx) continue;
+            const size_t g = (size_t)gj * nx + gi;
+            Pc[g] = sA[0][r][c + 1];
+            Pc[N + g] = sA[3][r][c + 1];
+        }
+        __syncthreads();
+    };
 
     auto loadW = [&](int r, int c, float* w) {
         w[0] = (float)sh1[r][c];
@@ -272,6 +290,7 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
             }
         }
         __syncthreads();
+        if (RECOMP) store_partials(par);
     }
 
     // x-edge classes C3 then C4
@@ -329,6 +348,20 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
             }
         }
         __syncthreads();
+        if (RECOMP && par == 0) store_partials(2);
+    }
+
+    if (RECOMP) {
+#pragma unroll 1
+        for (int e = tid; e < TX * TY; e += NT) {
+            const int r = e / TX, c = e % TX;
+            const int gj = j0 + r, gi = i0 + c;
+            if (gj >= ny || gi >= nx) continue;
+            const size_t g = (size_t)gj * nx + gi;
+#pragma unroll
+            for (int f = 0; f < 6; f++) ACC[f * N + g] = sA[f][r][c + 1];
+        }
+        return 0.0;
     }
 
     // compensated cell update of the tile's own cells
@@ -348,7 +381,6 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
     kk.ieps = (float)(1.0 / k.eps_h);
     kk.ten_eps = kc.ten_eps;
     double Amax = 0.0;
-    const double tnow = ctl->t;
     for (int e = tid; e < TX * TY; e += NT) {
         const int r = e / TX, c = e % TX;
         const int gj = j0 + r, gi = i0 + c;
@@ -414,11 +446,29 @@ __global__ void __launch_bounds__(NT) k_fused(Consts k, const float* __restrict_
     }
     const int tx = b % tiles_x, ty = b / tiles_x;
     const double dt = step_dt(a.ctl, a.dt_seq);
-    const double Amax = tile_step<TX, TY, NT>(k, W, Wn, H, dt, a.ctl, a.emax, tx * TX, ty * TY);
+    const double Amax = tile_step<TX, TY, NT>(k, W, Wn, H, dt, a.ctl->t, a.emax, tx * TX, ty * TY);
     block_max_atomic<NT>(Amax, &a.ctl->maxA_next);
 }
 
 constexpr int FTX = 62, FTY = 14, FNT = 512;
+
+// Forward recomputation for the discrete adjoint: the step's partial and
+// final flux sums at state W with the recorded dt of step `step`.
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_recompute(Consts k, const float* __restrict__ W, const double* __restrict__ H,
+                                                  const double* __restrict__ dt_seq, int step, float* __restrict__ P,
+                                                  float* __restrict__ ACC, int tiles_x, int tiles_y)
+{
+    const int nb = tiles_x * tiles_y;
+    int b;
+    {
+        const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+        const int q = nb >> 3, rem = nb & 7;
+        b = xcd * q + (xcd < rem ? xcd : rem) + slot;
+    }
+    const int tx = b % tiles_x, ty = b / tiles_x;
+    tile_step<TX, TY, NT, true>(k, W, nullptr, H, dt_seq[step], 0.0, nullptr, tx * TX, ty * TY, P, ACC);
+}
 
 // ------------------------------------------------------------- ensemble step
 // One launch steps every scenario of a batch.  tab[blockIdx.x] = (scenario
@@ -442,7 +492,7 @@ __global__ void __launch_bounds__(NT) k_batch(const BatchScen* __restrict__ sd, 
     const int tx = tile % d.tiles_x, ty = tile / d.tiles_x;
     const int cur = step & 1;
     const Consts k = d.k;
-    const double Amax = tile_step<TX, TY, NT>(k, d.W[cur], d.W[cur ^ 1], d.H, ctl->dt_cur, ctl, d.emax,
+    const double Amax = tile_step<TX, TY, NT>(k, d.W[cur], d.W[cur ^ 1], d.H, ctl->dt_cur, ctl->t, d.emax,
                                               tx * TX, ty * TY);
     block_max_atomic<NT>(Amax, &ctl->maxA_next);
 }
@@ -497,6 +547,16 @@ hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const dou
                        dim3(f32::FNT), 0, s, k, W, Wn, H, a, tiles_x, tiles_y);
     if (a.ev1) hipEventRecord(a.ev1, s);
     hipLaunchKernelGGL(f32::k_finalize, dim3(1), dim3(64), 0, s, k, Wn, H, a);
+    return hipGetLastError();
+}
+
+hipError_t f32c_recompute(const Consts& k, const float* W, const double* H, const double* dt_seq, int step, float* P,
+                          float* ACC, hipStream_t s)
+{
+    const int tiles_x = (k.nx + f32::FTX - 1) / f32::FTX;
+    const int tiles_y = (k.ny + f32::FTY - 1) / f32::FTY;
+    hipLaunchKernelGGL((f32::k_recompute<f32::FTX, f32::FTY, f32::FNT>), dim3((unsigned)(tiles_x * tiles_y)),
+                       dim3(f32::FNT), 0, s, k, W, H, dt_seq, step, P, ACC, tiles_x, tiles_y);
     return hipGetLastError();
 }
 

@@ -251,6 +251,23 @@ __global__ void __launch_bounds__(256) k_update(Consts k, const double* __restri
 }
 
 // ----------------------------------------------------------------- fused path
+// Both cells of an edge without sediment (h2 = q2x = q2y = 0).  A wavefront
+// whose edges all qualify takes edge1(), which yields no layer-2 fluxes, so
+// the layer-2 accumulators are left as they are.
+__device__ __forceinline__ bool no_sediment(const double* W0, const double* W1)
+{
+    return (W0[3] == 0.0) && (W0[4] == 0.0) && (W0[5] == 0.0) && (W1[3] == 0.0) && (W1[4] == 0.0) && (W1[5] == 0.0);
+}
+
+// sA[f][r][c] -= F[f] for the first NF fields, wave-speed sum += lam
+template <int NF, int TY, int CW>
+__device__ __forceinline__ void accumulate(double (&sA)[7][TY][CW], int r, int c, const double* F, double lam)
+{
+#pragma unroll
+    for (int f = 0; f < NF; f++) sA[f][r][c] -= F[f];
+    sA[6][r][c] += lam;
+}
+
 // Tile of TX x TY cells (both even), NT threads.  LDS planes:
 //   sW[f][r][c]  f = h1,q1x,q1y,h2,q2x,q2y,H ; r = tile row + 1 ; c = tile col + 1
 //   sA[f][r][c]  f = dh1..dq2y, A            ; r = tile row     ; c = tile col + 1
@@ -330,19 +347,20 @@ __global__ void __launch_bounds__(NT) k_fused(Consts k, const double* __restrict
                 interior = true;
             }
             const bool has0 = (r0 >= 0 && r0 < TY), has1 = interior && (r0 + 1 < TY);
-            const double a0h1 = has0 ? sA[0][r0][c] : 0.0, a0h2 = has0 ? sA[3][r0][c] : 0.0;
-            if (has1) { a1h1 = sA[0][r0 + 1][c]; a1h2 = sA[3][r0 + 1][c]; }
+            const double a0h1 = has0 ? sA[0][r0][c] : 0.0;
+            if (has1) a1h1 = sA[0][r0 + 1][c];
             EdgeFlux F;
-            if (edge(k, W0, W1, H0, H1, 0.0, n1y, k.dx, dt, a0h1, a0h2, a1h1, a1h2, interior, F)) {
-                if (has0) {
-#pragma unroll
-                    for (int f = 0; f < 6; f++) sA[f][r0][c] -= F.Fm[f];
-                    sA[6][r0][c] += F.lam;
+            if (__all(no_sediment(W0, W1))) {
+                if (edge1(k, W0, W1, H0, H1, 0.0, n1y, k.dx, dt, a0h1, a1h1, interior, F)) {
+                    if (has0) accumulate<3>(sA, r0, c, F.Fm, F.lam);
+                    if (has1) accumulate<3>(sA, r0 + 1, c, F.Fp, F.lam);
                 }
-                if (has1) {
-#pragma unroll
-                    for (int f = 0; f < 6; f++) sA[f][r0 + 1][c] -= F.Fp[f];
-                    sA[6][r0 + 1][c] += F.lam;
+            } else {
+                const double a0h2 = has0 ? sA[3][r0][c] : 0.0;
+                if (has1) a1h2 = sA[3][r0 + 1][c];
+                if (edge(k, W0, W1, H0, H1, 0.0, n1y, k.dx, dt, a0h1, a0h2, a1h1, a1h2, interior, F)) {
+                    if (has0) accumulate<6>(sA, r0, c, F.Fm, F.lam);
+                    if (has1) accumulate<6>(sA, r0 + 1, c, F.Fp, F.lam);
                 }
             }
         }
@@ -379,17 +397,20 @@ __global__ void __launch_bounds__(NT) k_fused(Consts k, const double* __restrict
                 n1x = 1.0;
                 interior = true;
             }
-            const double a0h1 = sA[0][r][c0], a0h2 = sA[3][r][c0];
-            if (interior) { a1h1 = sA[0][r][c0 + 1]; a1h2 = sA[3][r][c0 + 1]; }
+            const double a0h1 = sA[0][r][c0];
+            if (interior) a1h1 = sA[0][r][c0 + 1];
             EdgeFlux F;
-            if (edge(k, W0, W1, H0, H1, n1x, 0.0, k.dy, dt, a0h1, a0h2, a1h1, a1h2, interior, F)) {
-#pragma unroll
-                for (int f = 0; f < 6; f++) sA[f][r][c0] -= F.Fm[f];
-                sA[6][r][c0] += F.lam;
-                if (interior) {
-#pragma unroll
-                    for (int f = 0; f < 6; f++) sA[f][r][c0 + 1] -= F.Fp[f];
-                    sA[6][r][c0 + 1] += F.lam;
+            if (__all(no_sediment(W0, W1))) {
+                if (edge1(k, W0, W1, H0, H1, n1x, 0.0, k.dy, dt, a0h1, a1h1, interior, F)) {
+                    accumulate<3>(sA, r, c0, F.Fm, F.lam);
+                    if (interior) accumulate<3>(sA, r, c0 + 1, F.Fp, F.lam);
+                }
+            } else {
+                const double a0h2 = sA[3][r][c0];
+                if (interior) a1h2 = sA[3][r][c0 + 1];
+                if (edge(k, W0, W1, H0, H1, n1x, 0.0, k.dy, dt, a0h1, a0h2, a1h1, a1h2, interior, F)) {
+                    accumulate<6>(sA, r, c0, F.Fm, F.lam);
+                    if (interior) accumulate<6>(sA, r, c0 + 1, F.Fp, F.lam);
                 }
             }
         }

@@ -246,6 +246,135 @@ __device__ __forceinline__ bool edge(const Consts& k, const double* W0, const do
     return true;
 }
 
+// The compare-swap network of eig() when the internal pair is zero
+// (Dy = Dz = 0); aut2 (if not null) gets max(|Dy|, |Dz|) of the result.
+__device__ __forceinline__ void net_zero_pair(double& Dx, double& Dw, double* aut2)
+{
+    double Dy = 0.0, Dz = 0.0, t;
+    if (Dx > Dy) { t = Dy; Dy = Dx; Dx = t; }
+    if (Dx > Dz) { t = Dz; Dz = Dx; Dx = t; }
+    if (Dx > Dw) { t = Dw; Dw = Dx; Dx = t; }
+    if (Dy > Dz) { t = Dz; Dz = Dy; Dy = t; }
+    if (Dy > Dw) { t = Dw; Dw = Dy; Dy = t; }
+    if (Dz > Dy) { t = Dy; Dy = Dz; Dz = t; }   // the reference's inverted (2,1) step
+    if (Dz > Dw) { t = Dw; Dw = Dz; Dz = t; }
+    if (aut2) *aut2 = fmax(fabs(Dy), fabs(Dz));
+}
+
+// edge() for an edge whose two cells carry no sediment (h2 = 0, q2 = 0 on
+// both sides).  Terms that such an edge makes zero are dropped and every
+// other operation keeps edge()'s order, so without contraction the layer-1
+// fluxes and the wave speed equal edge()'s (zero results may differ in
+// sign); the layer-2 fluxes are zero and are not produced (Fm[3..5],
+// Fp[3..5] unset).  With h2ij = 0 the Coulomb flag is false for either
+// friction law, deta2 = 0, the internal eigenvalue pair is (0, 0) and the PVM
+// takes its two-wave or Rusanov form.
+__device__ __forceinline__ bool edge1(const Consts& k, const double* W0, const double* W1, double H0, double H1,
+                                      double n1x, double n1y, double len, double dt, double a0h1, double a1h1,
+                                      bool interior, EdgeFlux& out)
+{
+    const double g = k.g, eps = k.eps_h, area = k.area;
+    double w00 = W0[0], w01 = W0[1] * n1x + W0[2] * n1y;
+    double w10 = W1[0], w11 = W1[1] * n1x + W1[2] * n1y;
+
+    // dry/wet treatment (src/GPU/Arista_kernel.cu:301-321) with h2 = 0
+    if ((w00 < eps) && (H1 - w10 > H0)) w11 = 0.0;
+    if ((w10 < eps) && (H0 - w00 > H1)) w01 = 0.0;
+
+    double h1ij = 0.5 * (w00 + w10);
+    double u0n = desing_u(w00, w01, eps), u1n = desing_u(w10, w11, eps);
+    double s0 = sqrt(w00), s1 = sqrt(w10);
+    double u1ij = (s0 * u0n + s1 * u1n) / (s0 + s1 + LH_EPS);
+    if (!(h1ij >= LH_EPS)) return false;
+
+    double Hm = fmin(H0, H1);
+    double e0 = fmax(w00 - H0 + Hm, 0.0), e1 = fmax(w10 - H1 + Hm, 0.0);
+    double deta1 = e1 - e0;
+    double tp1 = g * h1ij * deta1;
+
+    // wave speeds (src/GPU/Arista_kernel.cu:427-444), internal pair (0, 0)
+    double Dx, Dw, aut2, Lx, Lw, Rx, Rw;
+    {
+        double q = h1ij * u1ij, h = h1ij;
+        double hd = sqrt(p4(h) + p4(fmax(h, eps)));
+        double u = LH_SQRT2 * q * h / hd, cg = sqrt(g * h);
+        Dx = u - cg;
+        Dw = u + cg;
+        net_zero_pair(Dx, Dw, &aut2);
+        h = w00;
+        hd = sqrt(p4(h) + p4(fmax(h, eps)));
+        u = LH_SQRT2 * w01 * h / hd;
+        cg = sqrt(g * h);
+        Lx = u - cg;
+        Lw = u + cg;
+        net_zero_pair(Lx, Lw, nullptr);
+        h = w10;
+        hd = sqrt(p4(h) + p4(fmax(h, eps)));
+        u = LH_SQRT2 * w11 * h / hd;
+        cg = sqrt(g * h);
+        Rx = u - cg;
+        Rw = u + cg;
+        net_zero_pair(Rx, Rw, nullptr);
+    }
+    double aut1 = fmin(Dx, fmin(Lx, Rx));
+    double aut3 = fmax(Dw, fmax(Lw, Rw));
+    double maxl = fmax(fabs(aut3), fabs(aut1));
+    double a = fabs(u1ij);
+    if (a > maxl) maxl = a;
+    double sg = aut1 + aut3;
+    if (!(fabs(sg) < LH_EPS) && !(sg > 0)) aut2 = -aut2;
+
+    // PVM coefficients (src/GPU/Arista_kernel.cu:446-465), h2ij < eps
+    double c0, c1;
+    if ((fabs(aut1 - aut3) >= LH_EPS) && (fabs(aut1 - aut2) >= LH_EPS) && (fabs(aut2 - aut3) >= LH_EPS)) {
+        c0 = (aut3 * fabs(aut1) - aut1 * fabs(aut3)) / (aut3 - aut1);
+        c1 = (fabs(aut3) - fabs(aut1)) / (aut3 - aut1);
+    } else {
+        c0 = maxl;
+        c1 = 0.0;
+    }
+
+    double FL0 = w01, FL1 = (w00 < LH_EPS) ? 0.0 : desing_u(w00, w01, eps) * w01;
+    double FR0 = w11, FR1 = (w10 < LH_EPS) ? 0.0 : desing_u(w10, w11, eps) * w11;
+    double dF0 = FR0 - FL0, dF1 = FR1 - FL1;
+    double t20 = dF0, t21 = tp1 + dF1;
+    double M0 = 0.5 * dF0, M1 = 0.5 * (dF1 + tp1);
+    double I0 = deta1, I1 = w11 - w01;
+    double V0 = 0.5 * (c0 * I0 + c1 * t20);
+    double V1 = 0.5 * (c0 * I1 + c1 * t21);
+    double Fm0 = M0 + (FL0 - V0), Fm1 = M1 + (FL1 - V1);
+    double Fp0 = M0 + (V0 - FR0), Fp1 = M1 + (V1 - FR1);
+
+    // tangential upwinding (src/GPU/Arista_kernel.cu:506-532)
+    double q0t = W0[2] * n1x - W0[1] * n1y, q1t = W1[2] * n1x - W1[1] * n1y;
+    double ut1 = (fabs(Fm0) < LH_EPS) ? 0.0 : ((Fm0 > 0) ? desing_u(w00, q0t, eps) : desing_u(w10, q1t, eps));
+    double T1 = Fm0 * ut1;
+    double* P = out.Fp;
+    double* Mn = out.Fm;
+    P[0] = len * Fp0; P[1] = len * (Fp1 * n1x + T1 * n1y); P[2] = len * (Fp1 * n1y - T1 * n1x);
+    Mn[0] = len * Fm0; Mn[1] = len * (Fm1 * n1x - T1 * n1y); Mn[2] = len * (Fm1 * n1y + T1 * n1x);
+
+    // positivity with the partial accumulators (src/GPU/Arista_kernel.cu:554-609)
+    const double factor = 1.0 * k.beta;
+    double bq = dt / area;
+    double hp00 = W0[0] + bq * a0h1, hp10 = 0.0;
+    if (interior) hp10 = W1[0] + bq * a1h1;
+    double d1 = LH_BIG, d2 = LH_BIG;
+    if (Mn[0] > 0.0) d1 = hp00 * area / (factor * Mn[0] + LH_EPS);
+    if (interior && P[0] > 0.0) d2 = hp10 * area / (factor * P[0] + LH_EPS);
+    double dt0 = fmin(d1, d2);
+    double al = (dt <= dt0) ? 1.0 : dt0 / (dt + LH_EPS);
+    Mn[0] = al * Mn[0]; Mn[1] = al * Mn[1]; Mn[2] = al * Mn[2];
+    P[0] = al * P[0]; P[1] = al * P[1]; P[2] = al * P[2];
+
+    if (maxl < eps) maxl += eps;
+    out.lam = len * maxl / k.peso;
+    if (k.peso != 1.0) {
+        for (int i = 0; i < 3; i++) { Mn[i] = k.peso * Mn[i]; P[i] = k.peso * P[i]; }
+    }
+    return true;
+}
+
 // Initial wave-speed edge (src/GPU/Arista_kernel.cu:1040-1127).
 __device__ __forceinline__ double edge_dt0(const Consts& k, const double* W0, const double* W1, double n1x,
                                            double n1y, double len)

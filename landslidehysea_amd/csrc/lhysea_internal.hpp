@@ -57,6 +57,10 @@ hipError_t f64_step_fused(const Consts& k, const double* W, double* Wn, const do
 hipError_t f32c_initial_dt(const Consts& k, const float* W, Ctl* ctl, hipStream_t s);
 hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const double* H, const StepArgs& a,
                            hipStream_t s);
+// partial h sums read by classes 2..4 (P, [3][2][N]) and flux sums (ACC, [6][N])
+// of the step from W with dt = dt_seq[step], by the fused tile kernel
+hipError_t f32c_recompute(const Consts& k, const float* W, const double* H, const double* dt_seq, int step, float* P,
+                          float* ACC, hipStream_t s);
 
 // One scenario of an ensemble batch as the batched kernels read it (device copy).
 struct BatchScen {
@@ -85,6 +89,11 @@ hipError_t f32c_batch_step(int tile, const BatchScen* sd, int nscen, const unsig
 // Adjoint (lhysea_adjoint.hip)
 hipError_t adj_reverse_step_f32(const Consts& k, const float* W, const double* H, float* P, float* ACC, float* AB,
                                 double* LAM, double* gmu, const double* dt_hist, int step, hipStream_t s);
+// FP32 reverse step as two launches: the recomputation and one tiled kernel
+// for the cell and edge adjoints (lambda^{n+1} in LAMin, lambda^n to LAMout)
+hipError_t adj_reverse_step_f32_fused(const Consts& k, const float* W, const double* H, float* P, float* ACC,
+                                      const double* LAMin, double* LAMout, double* gmu, const double* dt_hist,
+                                      int step, hipStream_t s);
 hipError_t adj_reverse_step_f64(const Consts& k, const double* W, const double* H, double* P, double* ACC,
                                 double* AB, double* LAM, double* gmu, const double* dt_hist, int step, hipStream_t s);
 hipError_t adj_inject(double* LAM, size_t N, const int* gauges, int ng, const double* G, hipStream_t s);

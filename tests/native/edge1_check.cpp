@@ -5,13 +5,16 @@
 //    near-dry depths, steps in the bottom, interior and ghost (wall / open)
 //    edges in x and y;
 //  * cell_update1() must give cell_update()'s layer-1 state and cell_update()
-//    a zero layer-2 state, on random water-only cells with zero layer-2 sums.
+//    a zero layer-2 state, on random water-only cells with zero layer-2 sums;
+//  * the FP64 edge1() must give the FP64 edge()'s layer-1 fluxes and wave
+//    speed, for the Coulomb and the Pouliquen law.
 // Build: g++ -std=c++17 -O2 -ffp-contract=off -I tests/native edge1_check.cpp
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstdio>
 #include <random>
 #include "../../landslidehysea_amd/csrc/lhysea_fp32.hpp"
+#include "../../landslidehysea_amd/csrc/lhysea_fp64.hpp"
 using namespace lhysea::f32;
 
 static bool same(float a, float b) { return a == b; }
@@ -126,5 +129,63 @@ int main(int argc, char** argv)
                         p[2].hi);
     }
     std::printf("cells %ld mismatches %ld\n", iters, cbad);
-    return (bad + cbad) != 0;
+
+    long dbad = 0, dwet = 0;
+    for (long it = 0; it < iters; it++) {
+        lhysea::Consts k{};
+        k.r = 0.45 + 0.15 * U(rng);
+        k.g = 9.81;
+        k.eps_h = 5e-3;
+        k.area = 3091.0;
+        k.mu = 0.17;
+        k.peso = 1.0;
+        k.beta = 1.0;
+        k.friction_law = U(rng) < 0.3 ? 1 : 0;
+        k.tan1 = 0.1; k.tan2 = 0.3; k.tan3 = 0.12; k.tan4 = 0.35; k.LoverH = 1.0;
+        const double dt = 0.05 + 0.3 * U(rng);
+        const double H0 = 400 * U(rng);
+        double H1 = std::max(0.0, H0 + (U(rng) < 0.3 ? 0.0 : 20 * (U(rng) - 0.5)));
+        auto depth = [&](double H) {
+            const double r = U(rng);
+            return r < 0.15 ? 0.0 : r < 0.3 ? 0.01 * U(rng) : r < 0.4 ? 1e-4 * U(rng) : H * U(rng);
+        };
+        double W0[6] = {depth(H0), 0, 0, 0, 0, 0}, W1[6] = {depth(H1), 0, 0, 0, 0, 0};
+        for (int f = 1; f < 3; f++) {
+            W0[f] = (U(rng) - 0.5) * 20 * (W0[0] > 0);
+            W1[f] = (U(rng) - 0.5) * 20 * (W1[0] > 0);
+        }
+        const bool interior = U(rng) < 0.85;
+        double n1x = 1, n1y = 0, len = 55.6;
+        if (U(rng) < 0.5) {
+            n1x = 0;
+            n1y = 1;
+        }
+        if (!interior) {
+            const double bd = U(rng) < 0.5 ? -1.0 : 1.0;
+            for (int f = 0; f < 6; f++) W1[f] = W0[f];
+            if (n1x != 0) W1[1] *= bd;
+            else W1[2] *= bd;
+            H1 = H0;
+            if (U(rng) < 0.5) {
+                n1x = -n1x;
+                n1y = -n1y;
+            }
+        }
+        const double a0 = (U(rng) - 0.5) * 100, a1 = (U(rng) - 0.5) * 100;
+        lhysea::f64::EdgeFlux A, B;
+        const bool wa = lhysea::f64::edge(k, W0, W1, H0, H1, n1x, n1y, len, dt, a0, 0.0, a1, 0.0, interior, A);
+        const bool wb = lhysea::f64::edge1(k, W0, W1, H0, H1, n1x, n1y, len, dt, a0, a1, interior, B);
+        bool ok = wa == wb;
+        if (ok && wa) {
+            ok = A.lam == B.lam;
+            for (int f = 0; f < 3; f++) ok = ok && A.Fm[f] == B.Fm[f] && A.Fp[f] == B.Fp[f];
+            for (int f = 3; f < 6; f++) ok = ok && A.Fm[f] == 0.0 && A.Fp[f] == 0.0;
+        }
+        dwet += wa;
+        if (!ok && ++dbad <= 5)
+            std::printf("fp64 mismatch at %ld: Fm %g %g %g / %g %g %g, lam %g / %g\n", it, A.Fm[0], A.Fm[1],
+                        A.Fm[2], B.Fm[0], B.Fm[1], B.Fm[2], A.lam, B.lam);
+    }
+    std::printf("fp64 edges %ld wet %ld mismatches %ld\n", iters, dwet, dbad);
+    return (bad + cbad + dbad) != 0;
 }
