@@ -28,6 +28,11 @@ __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x);
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float fexp(float x) { return __expf(x); }
+// x^y for x >= 0 as exp2(y log2 x) with the hardware log2 / exp2 (x = 0 gives 0)
+__device__ __forceinline__ float fpow(float x, float y)
+{
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
 
 // s(h): U(h,q) = q * s(h) = sqrt2 h / sqrt(h^4 + max(h,eps)^4)  (= 1/h for h >= eps);
 // one reciprocal square root, no branch
@@ -239,20 +244,17 @@ __device__ __forceinline__ bool edge(const K32& k, const float* W0, const float*
     return true;
 }
 
-// Sorting network of eig() for a state whose internal pair is zero
-// (Dy = Dz = 0): returns the network's Dx (the minimum) and Dw outputs and
-// max(|Dy|, |Dz|) of its final Dy, Dz.
+// Sorting network of eig() for a state whose internal pair is zero: on
+// (Dx, 0, 0, Dw) with Dx <= Dw (Dx = u - c, Dw = u + c, c >= 0) the seven
+// compare-swaps, the inverted (2,1) step included, give Dx' = min(Dx, 0),
+// Dw' = max(Dw, 0) and an inner pair with max(|Dy'|, |Dz'|) = max(Dx, -Dw, 0)
+// (case by case: Dx >= 0 gives (0, Dx, 0, Dw); Dw <= 0 gives (Dx, 0, Dw, 0);
+// otherwise (Dx, 0, 0, Dw)).
 __device__ __forceinline__ void net_zero_pair(float& Dx, float& Dw, float* aut2)
 {
-    float Dy = 0.0f, Dz = 0.0f, lo, hi;
-    lo = fminf(Dx, Dy); hi = fmaxf(Dx, Dy); Dx = lo; Dy = hi;
-    lo = fminf(Dx, Dz); hi = fmaxf(Dx, Dz); Dx = lo; Dz = hi;
-    lo = fminf(Dx, Dw); hi = fmaxf(Dx, Dw); Dx = lo; Dw = hi;
-    lo = fminf(Dy, Dz); hi = fmaxf(Dy, Dz); Dy = lo; Dz = hi;
-    lo = fminf(Dy, Dw); hi = fmaxf(Dy, Dw); Dy = lo; Dw = hi;
-    lo = fminf(Dy, Dz); hi = fmaxf(Dy, Dz); Dy = hi; Dz = lo;   // inverted (2,1) step
-    lo = fminf(Dz, Dw); hi = fmaxf(Dz, Dw); Dz = lo; Dw = hi;
-    if (aut2) *aut2 = fmaxf(fabsf(Dy), fabsf(Dz));
+    if (aut2) *aut2 = fmaxf(fmaxf(Dx, -Dw), 0.0f);
+    Dx = fminf(Dx, 0.0f);
+    Dw = fmaxf(Dw, 0.0f);
 }
 
 // edge() for an edge whose two cells carry no sediment (h2 = 0 and q2 = 0 on
@@ -497,7 +499,7 @@ __device__ __forceinline__ void cell_update(const KCell& k, Pair* o, const float
             const float du = fsqrt(ddx * ddx + ddy * ddy), uo2 = fsqrt(uo2x * uo2x + uo2y * uo2y);
             float c1 = dt * h2 * phi * k.mfc * du;
             float c2 = dt * r * h1 * phi * k.mfc * du;
-            float c3 = dt * k.g * k.mfs * k.mfs * uo2 * rcp(__powf(h2, 4.0f / 3.0f) + LH_EPSF);
+            float c3 = dt * k.g * k.mfs * k.mfs * uo2 * rcp(fpow(h2, 4.0f / 3.0f) + LH_EPSF);
             float det = rcp((1.0f + c1) * (1.0f + c2 + c3) - c1 * c2);
             float a11 = h1 * s1 * (1.0f + c2 + c3) * det, a12 = h1 * c1 * s2 * det;
             float a21 = h2 * c2 * s1 * det, a22 = h2 * s2 * (1.0f + c1) * det;
@@ -507,14 +509,14 @@ __device__ __forceinline__ void cell_update(const KCell& k, Pair* o, const float
         }
         if ((h1 > 0.0f) && (h2 < eps)) {
             const float uo1 = fsqrt(uo1x * uo1x + uo1y * uo1y);
-            float c = dt * k.g * k.mf0 * k.mf0 * uo1 * rcp(__powf(h1, 4.0f / 3.0f) + LH_EPSF);
+            float c = dt * k.g * k.mf0 * k.mf0 * uo1 * rcp(fpow(h1, 4.0f / 3.0f) + LH_EPSF);
             float f = h1 * s1 * rcp(1.0f + c);
             n[1] = scale(n[1], f);
             n[2] = scale(n[2], f);
         }
         if ((h2 > 0.0f) && (h1 < eps)) {
             const float uo2 = fsqrt(uo2x * uo2x + uo2y * uo2y);
-            float c = dt * k.g * k.mfs * k.mfs * uo2 * rcp(__powf(h2, 4.0f / 3.0f) + LH_EPSF);
+            float c = dt * k.g * k.mfs * k.mfs * uo2 * rcp(fpow(h2, 4.0f / 3.0f) + LH_EPSF);
             float f = h2 * s2 * rcp(1.0f + c);
             n[4] = scale(n[4], f);
             n[5] = scale(n[5], f);
@@ -572,7 +574,7 @@ __device__ __forceinline__ void cell_update1(const KCell& k, Pair* o, const floa
         const float s1 = sdes(h1, eps, eps4);
         if (h1 > 0.0f) {
             const float uo1 = fsqrt(uo1x * uo1x + uo1y * uo1y);
-            float c = dt * k.g * k.mf0 * k.mf0 * uo1 * rcp(__powf(h1, 4.0f / 3.0f) + LH_EPSF);
+            float c = dt * k.g * k.mf0 * k.mf0 * uo1 * rcp(fpow(h1, 4.0f / 3.0f) + LH_EPSF);
             float f = h1 * s1 * rcp(1.0f + c);
             n[1] = scale(n[1], f);
             n[2] = scale(n[2], f);

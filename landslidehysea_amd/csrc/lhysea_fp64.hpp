@@ -246,19 +246,16 @@ __device__ __forceinline__ bool edge(const Consts& k, const double* W0, const do
     return true;
 }
 
-// The compare-swap network of eig() when the internal pair is zero
-// (Dy = Dz = 0); aut2 (if not null) gets max(|Dy|, |Dz|) of the result.
+// The compare-swap network of eig() when the internal pair is zero: on
+// (Dx, 0, 0, Dw) with Dx <= Dw (Dx = u - c, Dw = u + c, c >= 0) it gives
+// Dx' = min(Dx, 0), Dw' = max(Dw, 0) and an inner pair with
+// max(|Dy'|, |Dz'|) = max(Dx, -Dw, 0) (Dx >= 0: (0, Dx, 0, Dw); Dw <= 0:
+// (Dx, 0, Dw, 0); otherwise (Dx, 0, 0, Dw)); aut2 (if not null) gets it.
 __device__ __forceinline__ void net_zero_pair(double& Dx, double& Dw, double* aut2)
 {
-    double Dy = 0.0, Dz = 0.0, t;
-    if (Dx > Dy) { t = Dy; Dy = Dx; Dx = t; }
-    if (Dx > Dz) { t = Dz; Dz = Dx; Dx = t; }
-    if (Dx > Dw) { t = Dw; Dw = Dx; Dx = t; }
-    if (Dy > Dz) { t = Dz; Dz = Dy; Dy = t; }
-    if (Dy > Dw) { t = Dw; Dw = Dy; Dy = t; }
-    if (Dz > Dy) { t = Dy; Dy = Dz; Dz = t; }   // the reference's inverted (2,1) step
-    if (Dz > Dw) { t = Dw; Dw = Dz; Dz = t; }
-    if (aut2) *aut2 = fmax(fabs(Dy), fabs(Dz));
+    if (aut2) *aut2 = fmax(fmax(Dx, -Dw), 0.0);
+    Dx = fmin(Dx, 0.0);
+    Dw = fmax(Dw, 0.0);
 }
 
 // edge() for an edge whose two cells carry no sediment (h2 = 0, q2 = 0 on

@@ -19,3 +19,5 @@ inline float __builtin_amdgcn_sqrtf(float x) { return std::sqrt(x); }
 inline float __builtin_amdgcn_rsqf(float x) { return 1.0f / std::sqrt(x); }
 inline float __expf(float x) { return std::exp(x); }
 inline float __powf(float a, float b) { return std::pow(a, b); }
+inline float __builtin_amdgcn_logf(float x) { return std::log2(x); }
+inline float __builtin_amdgcn_exp2f(float x) { return std::exp2(x); }
