@@ -245,6 +245,11 @@ def main():
         adj = {}
         if fp32c:
             adj["fp32c"] = time_adjoint(L.FP32C, args.adjoint_steps, args.adjoint_snapshots)
+            # every state kept in HBM (1024^2 x 4000 steps: 201 GB of the 288 GB)
+            try:
+                adj["fp32c_all_states"] = time_adjoint(L.FP32C, args.adjoint_steps, args.adjoint_steps + 1)
+            except L.LhyseaError as e:
+                adj["fp32c_all_states"] = {"error": str(e)}
         if fp64:
             adj["fp64"] = time_adjoint(L.FP64, args.adjoint_steps, args.adjoint_snapshots)
         out["adjoint"] = adj

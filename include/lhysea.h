@@ -111,14 +111,16 @@ int lhysea_set_kernel(lhysea_sim *s, int32_t mode);
  * (bracketed launch by launch in a second pass when kernel_ms is not NULL). */
 int lhysea_time_steps(lhysea_sim *s, int32_t nsteps, double *total_ms, double *kernel_ms);
 
-/* Discrete adjoint (no counterpart in the reference; the north-star
- * extension of the step).  Runs nsteps forward steps from the freshly
+/* Discrete adjoint (no counterpart in the reference: an extension of the
+ * step for gradient-based inversion).  Runs nsteps forward steps from the freshly
  * uploaded state with a gauge-misfit loss
  *     L = sum_n sum_g (eta_g^n - obs[n*ng + g])^2,  eta = h1 + h2 - H after step n,
  * then the reverse sweep with binomial checkpointing over nsnap state
  * snapshots in HBM (branches and the dt sequence frozen at the forward
- * choices).  Outputs: the loss, dL/dW0 (6 planes like lhysea_upload; may be
- * NULL), dL/dmu of the Coulomb coefficient mu = |tan(angle1)| L/H and dL/dangle1.
+ * choices); with nsnap > nsteps every state is kept in HBM instead (nsteps + 1
+ * states, no step re-advanced).  Outputs: the loss, dL/dW0 (6 planes like
+ * lhysea_upload; may be NULL), dL/dmu of the Coulomb coefficient
+ * mu = |tan(angle1)| L/H and dL/dangle1.
  * stats (6 doubles, may be NULL): forward ms, reverse ms, re-advanced steps,
  * reverse steps, snapshots, optimal re-advance count.  Afterwards the handle
  * must be uploaded again before further steps. */

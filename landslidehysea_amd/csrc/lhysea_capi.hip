@@ -110,7 +110,7 @@ static StepArgs step_args(lhysea_sim* s)
     return a;
 }
 
-static int launch_steps(lhysea_sim* s, int nsteps, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+static StepArgs launch_args(lhysea_sim* s, hipEvent_t ev0, hipEvent_t ev1)
 {
     StepArgs a = step_args(s);
     a.ev0 = ev0;
@@ -121,17 +121,30 @@ static int launch_steps(lhysea_sim* s, int nsteps, hipEvent_t ev0 = nullptr, hip
         a.gseries = s->gseries - (size_t)s->gstart * s->ng;
         a.gcap = s->gstart + s->gcap;
     }
+    return a;
+}
+
+// One step from state buffer W to Wn (either precision's layout).
+static hipError_t launch_one(lhysea_sim* s, const void* W, void* Wn, const StepArgs& a)
+{
+    if (s->p.precision == LHYSEA_FP64) {
+        if (s->mode == MODE_FUSED) return f64_step_fused(s->k, (const double*)W, (double*)Wn, s->b64.H, a, s->stream);
+        return f64_step_fourpass(s->k, (const double*)W, (double*)Wn, s->b64.H, s->b64.acc, a, s->stream);
+    }
+    return f32c_step_fused(s->k, (const float*)W, (float*)Wn, s->b32.H, a, s->stream);
+}
+
+static void* state_buf(lhysea_sim* s, int b)
+{
+    return s->p.precision == LHYSEA_FP64 ? (void*)s->b64.W[b] : (void*)s->b32.W[b];
+}
+
+static int launch_steps(lhysea_sim* s, int nsteps, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+{
+    const StepArgs a = launch_args(s, ev0, ev1);
     for (int n = 0; n < nsteps; n++) {
-        hipError_t e;
         const int nxt = 1 - s->cur;
-        if (s->p.precision == LHYSEA_FP64) {
-            if (s->mode == MODE_FUSED)
-                e = f64_step_fused(s->k, s->b64.W[s->cur], s->b64.W[nxt], s->b64.H, a, s->stream);
-            else
-                e = f64_step_fourpass(s->k, s->b64.W[s->cur], s->b64.W[nxt], s->b64.H, s->b64.acc, a, s->stream);
-        } else {
-            e = f32c_step_fused(s->k, s->b32.W[s->cur], s->b32.W[nxt], s->b32.H, a, s->stream);
-        }
+        hipError_t e = launch_one(s, state_buf(s, s->cur), state_buf(s, nxt), a);
         if (e != hipSuccess) return hip_fail(e, "step launch");
         s->cur = nxt;
     }
@@ -474,6 +487,7 @@ struct AdjCtx {
     double* LAM;           // [6][N] state adjoint lambda of the current step
     double* LAM2;          // [6][N] second lambda buffer of the fused FP32 reverse step
     double* gmu;
+    const char* W;         // stored state of the reverse step (all states kept), else null
     int* gauges;
     double* gser;          // gauge series of the forward pass
     double* G;             // [nsteps][ng] loss gradient wrt eta
@@ -509,12 +523,14 @@ hipError_t hook_reverse(void* c, int k)
     AdjCtx* a = (AdjCtx*)c;
     lhysea_sim* s = a->s;
     hipError_t e;
+    // the state W^k: the working buffer, or its stored slot
+    const void* W = a->W ? (const void*)a->W : state_ptr(s);
     if (s->p.precision == LHYSEA_FP64)
-        e = adj_reverse_step_f64(s->k, s->b64.W[s->cur], s->b64.H, (double*)a->P, (double*)a->ACC, (double*)a->AB,
+        e = adj_reverse_step_f64(s->k, (const double*)W, s->b64.H, (double*)a->P, (double*)a->ACC, (double*)a->AB,
                                  a->LAM, a->gmu, s->dt_hist, k, s->stream);
     else {
         // lambda^{n+1} in LAM, lambda^n into LAM2, then swap
-        e = adj_reverse_step_f32_fused(s->k, s->b32.W[s->cur], s->b32.H, (float*)a->P, (float*)a->ACC, a->LAM,
+        e = adj_reverse_step_f32_fused(s->k, (const float*)W, s->b32.H, (float*)a->P, (float*)a->ACC, a->LAM,
                                        a->LAM2, a->gmu, s->dt_hist, k, s->stream);
         std::swap(a->LAM, a->LAM2);
     }
@@ -543,6 +559,7 @@ int lhysea_adjoint(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const int32_t* 
     a.state_bytes = 48 * N;       // 6 doubles or 12 floats per cell
     a.ng = ng;
     int rc = LHYSEA_OK;
+    hipError_t he0 = hipSuccess;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     const double* saved_seq = s->dt_seq;
     const bool saved_emax = s->track_emax;
@@ -571,7 +588,8 @@ int lhysea_adjoint(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const int32_t* 
         s->gcap = 0;
         return code;
     };
-    if (hipMalloc(&a.snap, (size_t)nsnap * a.state_bytes) != hipSuccess || hipMalloc(&a.P, 6 * N * fsz) != hipSuccess ||
+    const size_t nslots = (nsnap > nsteps) ? (size_t)nsteps + 1 : (size_t)nsnap;
+    if (hipMalloc(&a.snap, nslots * a.state_bytes) != hipSuccess || hipMalloc(&a.P, 6 * N * fsz) != hipSuccess ||
         hipMalloc(&a.ACC, 6 * N * fsz) != hipSuccess || hipMalloc(&a.AB, 6 * N * fsz) != hipSuccess ||
         hipMalloc(&a.LAM, 6 * N * sizeof(double)) != hipSuccess ||
         (!f64 && hipMalloc(&a.LAM2, 6 * N * sizeof(double)) != hipSuccess) ||
@@ -597,19 +615,31 @@ int lhysea_adjoint(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const int32_t* 
         rc = lhysea_initial_dt(s, nullptr);
         if (rc) return done(rc);
     }
+    // with a snapshot per state (nsnap > nsteps) every state stays in HBM:
+    // the forward writes step k's result straight into slot k + 1 and the
+    // reverse sweep reads slot k in place (no re-advance, no copy)
+    const bool store_all = nsnap > nsteps;
     std::vector<std::pair<long, int>> chain;
-    revolve_chain(nsteps, nsnap, chain);
+    if (!store_all) revolve_chain(nsteps, nsnap, chain);
     hipEventRecord(e0, s->stream);
     if (hook_store(&a, 0) != hipSuccess) return done(fail(LHYSEA_ERR_HIP, "snapshot copy failed"));
-    long pos = 0;
-    for (auto& ch : chain) {
-        rc = launch_steps(s, (int)(ch.first - pos));
+    if (store_all) {
+        const StepArgs sa = launch_args(s, nullptr, nullptr);
+        for (long k = 0; k < nsteps; k++) {
+            he0 = launch_one(s, a.snap + (size_t)k * a.state_bytes, a.snap + (size_t)(k + 1) * a.state_bytes, sa);
+            if (he0 != hipSuccess) return done(hip_fail(he0, "step launch"));
+        }
+    } else {
+        long pos = 0;
+        for (auto& ch : chain) {
+            rc = launch_steps(s, (int)(ch.first - pos));
+            if (rc) return done(rc);
+            if (hook_store(&a, ch.second) != hipSuccess) return done(fail(LHYSEA_ERR_HIP, "snapshot copy failed"));
+            pos = ch.first;
+        }
+        rc = launch_steps(s, (int)(nsteps - pos));
         if (rc) return done(rc);
-        if (hook_store(&a, ch.second) != hipSuccess) return done(fail(LHYSEA_ERR_HIP, "snapshot copy failed"));
-        pos = ch.first;
     }
-    rc = launch_steps(s, (int)(nsteps - pos));
-    if (rc) return done(rc);
     hipEventRecord(e1, s->stream);
 
     // 2. loss and its gradient with respect to the gauge series
@@ -635,9 +665,19 @@ int lhysea_adjoint(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const int32_t* 
     if (adj_inject(a.LAM, N, a.gauges, ng, a.G + (size_t)(nsteps - 1) * ng, s->stream) != hipSuccess)
         return done(fail(LHYSEA_ERR_HIP, "loss injection failed"));
 
-    // 3. reverse sweep with binomial checkpointing
+    // 3. reverse sweep with binomial checkpointing (or over the stored states)
     AdjointHooks h{hook_advance, hook_store, hook_restore, hook_reverse, &a, 0, 0};
-    hipError_t he = run_treeverse(h, nsnap, nsteps);
+    hipError_t he = hipSuccess;
+    if (store_all) {
+        for (long k = nsteps - 1; k >= 0 && he == hipSuccess; k--) {
+            a.W = a.snap + (size_t)k * a.state_bytes;
+            he = hook_reverse(&a, (int)k);
+            h.reverses++;
+        }
+        a.W = nullptr;
+    } else {
+        he = run_treeverse(h, nsnap, nsteps);
+    }
     if (he != hipSuccess) return done(hip_fail(he, "adjoint sweep"));
     hipEventRecord(e2, s->stream);
     HIPCHK(hipStreamSynchronize(s->stream));

@@ -178,12 +178,14 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
                                             float* __restrict__ P = nullptr, float* __restrict__ ACC = nullptr)
 {
     static_assert(TX % 2 == 0 && TY % 2 == 0, "tile origins must stay even");
-    constexpr int CW = TX + 2, RW = TY + 2;
-    __shared__ double sH[RW][CW];
-    __shared__ double sh1[RW][CW];
-    __shared__ double sh2[RW][CW];
-    __shared__ float sq[4][RW][CW];
-    __shared__ float sA[7][TY][CW];
+    // LDS rows are padded to an odd length (CW + 1 words) so that the
+    // stride-2 column access of the x-edge classes spreads over all banks
+    constexpr int CW = TX + 2, RW = TY + 2, CWP = CW + 1;
+    __shared__ double sH[RW][CWP];
+    __shared__ double sh1[RW][CWP];
+    __shared__ double sh2[RW][CWP];
+    __shared__ float sq[4][RW][CWP];
+    __shared__ float sA[7][TY][CWP];
 
     const int nx = k.nx, ny = k.ny;
     const size_t N = (size_t)nx * ny;
@@ -293,12 +295,20 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
         if (RECOMP) store_partials(par);
     }
 
-    // x-edge classes C3 then C4
+    // x-edge classes C3 then C4.  Work items go in 16-lane segments of 2 rows x
+    // 8 edge columns, two column-adjacent segments per 32 lanes: with the odd
+    // row length a 32-lane FP32 access and a 16-lane FP64 access then touch
+    // distinct banks.
+    static_assert(TY % 2 == 0, "row pairs");
+    constexpr int NB = (TX / 2 + 1 + 7) / 8;   // blocks of 8 edge columns
 #pragma unroll 1
     for (int par = 0; par < 2; par++) {
         const int ncol = (par == 0) ? (TX / 2 + 1) : (TX / 2);
-        for (int e = tid; e < ncol * TY; e += NT) {
-            const int r = e / ncol, il = par + 2 * (e % ncol);
+        for (int e = tid; e < (TY / 2) * NB * 16; e += NT) {
+            const int seg = e >> 4, w = e & 15;
+            const int r = 2 * (seg / NB) + (w >> 3), j = 8 * (seg % NB) + (w & 7);
+            if (j >= ncol) continue;
+            const int il = par + 2 * j;
             const int gj = j0 + r, ie = i0 + il;
             if (gj >= ny || ie > nx) continue;
             float W0[6], W1[6];

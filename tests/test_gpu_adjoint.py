@@ -158,3 +158,22 @@ def test_checkpoint_schedule_is_binomial():
         say(f"N={N} S={S}: re-advances {res['advances']} optimal total {res['optimal_advances']}")
         assert res["reverse_steps"] == N
         assert 0 <= chain < N
+
+
+def test_all_states_kept_equals_checkpointing():
+    """nsnap > nsteps keeps every state in HBM (no re-advance): same loss and
+    bit-identical gradients as the binomial schedule."""
+    sc, W, H, prm = case(40, 30, angle=2.0, semi_axes=(8.0, 6.0), precision=L.FP32C)
+    gauges = gauge_cells(sc, 4, seed=3)
+    N = 24
+    obs = np.zeros((N, len(gauges)))
+    out = {}
+    for snaps in (5, N + 1):
+        with L.Simulation(prm) as sim:
+            sim.upload(W, H)
+            out[snaps] = sim.adjoint(N, snaps, gauges, obs)
+    a, b = out[5], out[N + 1]
+    say("advances", a["advances"], b["advances"])
+    assert b["advances"] == 0 and b["reverse_steps"] == N
+    assert a["loss"] == b["loss"]
+    assert np.array_equal(a["grad_W0"], b["grad_W0"]) and a["grad_mu"] == b["grad_mu"]
