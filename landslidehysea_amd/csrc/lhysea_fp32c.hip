@@ -192,18 +192,55 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
     const K32 kc = make_k32(k, dt);
     const int tid = threadIdx.x;
 
-    for (int e = tid; e < RW * CW; e += NT) {
-        const int r = e / CW, c = e % CW;
-        const int gj = j0 + r - 1, gi = i0 + c - 1;
-        if (gj >= 0 && gj < ny && gi >= 0 && gi < nx) {
-            const size_t g = (size_t)gj * nx + gi;
-            sH[r][c] = H[g];
-            sh1[r][c] = (double)W[g] + (double)W[6 * N + g];
-            sh2[r][c] = (double)W[3 * N + g] + (double)W[9 * N + g];
-            sq[0][r][c] = W[N + g];
-            sq[1][r][c] = W[2 * N + g];
-            sq[2][r][c] = W[4 * N + g];
-            sq[3][r][c] = W[5 * N + g];
+    // HBM loads are issued early and all at once: the q1 compensations the
+    // cell update needs, then every staging element of this thread, before the
+    // first LDS store waits on them
+    constexpr int CU_IT = (TX * TY + NT - 1) / NT, ST_IT = (RW * CW + NT - 1) / NT;
+    float lo1x[CU_IT], lo1y[CU_IT];
+    if (!RECOMP) {
+#pragma unroll
+        for (int it = 0; it < CU_IT; it++) {
+            const int e = tid + it * NT;
+            const int r = e / TX, c = e % TX;
+            lo1x[it] = lo1y[it] = 0.0f;
+            if (e < TX * TY && j0 + r < ny && i0 + c < nx) {
+                const size_t g = (size_t)(j0 + r) * nx + (i0 + c);
+                lo1x[it] = W[7 * N + g];
+                lo1y[it] = W[8 * N + g];
+            }
+        }
+    }
+    {
+        double vH[ST_IT], vh1[ST_IT], vh2[ST_IT];
+        float vq[ST_IT][4];
+        bool in[ST_IT];
+#pragma unroll
+        for (int it = 0; it < ST_IT; it++) {
+            const int e = tid + it * NT;
+            const int r = e / CW, c = e % CW;
+            const int gj = j0 + r - 1, gi = i0 + c - 1;
+            in[it] = e < RW * CW && gj >= 0 && gj < ny && gi >= 0 && gi < nx;
+            if (in[it]) {
+                const size_t g = (size_t)gj * nx + gi;
+                vH[it] = H[g];
+                vh1[it] = (double)W[g] + (double)W[6 * N + g];
+                vh2[it] = (double)W[3 * N + g] + (double)W[9 * N + g];
+                vq[it][0] = W[N + g];
+                vq[it][1] = W[2 * N + g];
+                vq[it][2] = W[4 * N + g];
+                vq[it][3] = W[5 * N + g];
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < ST_IT; it++) {
+            if (!in[it]) continue;
+            const int e = tid + it * NT;
+            const int r = e / CW, c = e % CW;
+            sH[r][c] = vH[it];
+            sh1[r][c] = vh1[it];
+            sh2[r][c] = vh2[it];
+#pragma unroll
+            for (int f = 0; f < 4; f++) sq[f][r][c] = vq[it][f];
         }
     }
     for (int e = tid; e < TY * CW; e += NT) {
@@ -391,7 +428,10 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
     kk.ieps = (float)(1.0 / k.eps_h);
     kk.ten_eps = kc.ten_eps;
     double Amax = 0.0;
-    for (int e = tid; e < TX * TY; e += NT) {
+#pragma unroll
+    for (int it = 0; it < CU_IT; it++) {
+        const int e = tid + it * NT;
+        if (e >= TX * TY) continue;
         const int r = e / TX, c = e % TX;
         const int gj = j0 + r, gi = i0 + c;
         if (gj >= ny || gi >= nx) continue;
@@ -399,8 +439,8 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
         Pair o[6];
         const double h1d = sh1[r + 1][c + 1], h2d = sh2[r + 1][c + 1];
         o[0].hi = (float)h1d; o[0].lo = (float)(h1d - (double)o[0].hi);
-        o[1] = Pair{sq[0][r + 1][c + 1], W[7 * N + g]};
-        o[2] = Pair{sq[1][r + 1][c + 1], W[8 * N + g]};
+        o[1] = Pair{sq[0][r + 1][c + 1], lo1x[it]};
+        o[2] = Pair{sq[1][r + 1][c + 1], lo1y[it]};
         float ac[6];
 #pragma unroll
         for (int f = 0; f < 6; f++) ac[f] = sA[f][r][c + 1];
