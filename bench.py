@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the scale factors of a plain
 # 256 MiB copy (profiles/fused_step_fp32c_2048x2048_counters.txt,
 # profiles/copy_256MiB_counter_calibration.txt)
-TRAFFIC_BYTES_FP32C = 440.3e6
+TRAFFIC_BYTES_FP32C = 406.0e6
 NX = NY = 2048
 SEED = 2048
 
@@ -86,6 +86,36 @@ def time_adjoint(precision, steps=4000, snaps=64, n=1024, ngauges=16):
             "reverse_ms": round(rev, 2), "cost_vs_forward": round((fwd + rev) / fwd, 3),
             "mcells_fwd_plus_adjoint": round(n * n * steps / ((fwd + rev) * 1e-3) / 1e6, 2),
             "re_advances": res["advances"], "wall_s": round(wall, 2)}
+
+
+def time_large(steps=100, adj_steps=200, snaps=100, n=4096):
+    """4096x4096 workload: three seeded slides of 20-80 m on the same shelf
+    construction; FP32C forward steps and the discrete adjoint of a bounded
+    prefix with `snaps` HBM snapshots (0.8 GB each)."""
+    sc = S.make_scenario(n, n, seed=n, n_slides=3, thickness=(20.0, 80.0))
+    H, _ = S.hmin_shift(sc.H)
+    W = np.ascontiguousarray(sc.state)
+    H = np.ascontiguousarray(H)
+    prm = L.Params(nx=n, ny=n, dx=sc.dx, dy=sc.dy, angle_deg=(10.0,), precision=L.FP32C)
+    with L.Simulation(prm) as sim:
+        sim.upload(W, H)
+        sim.initial_dt()
+        sim.step(10)
+        sim.sync()
+        total_ms, kernel_ms = sim.time_steps(steps, per_kernel=True)
+    gauges = S.gauge_cells(sc, 16, seed=16)
+    obs = np.zeros((adj_steps, len(gauges)))
+    with L.Simulation(prm) as sim:
+        sim.upload(W, H)
+        res = sim.adjoint(adj_steps, snaps, gauges, obs)
+    fwd, rev = res["forward_ms"], res["reverse_ms"]
+    N = n * n
+    return {"grid": f"{n}x{n}", "steps": steps, "mcells": round(N * steps / (total_ms * 1e-3) / 1e6, 2),
+            "kernel_ms": round(kernel_ms, 4),
+            "achieved_gbs": round(BYTES_PER_CELL * N / (kernel_ms * 1e-3) / 1e9, 1),
+            "adjoint": {"steps": adj_steps, "snapshots": snaps, "forward_ms": round(fwd, 2),
+                        "reverse_ms": round(rev, 2), "cost_vs_forward": round((fwd + rev) / fwd, 3),
+                        "re_advances": res["advances"]}}
 
 
 def time_ensemble(steps=200, warmup=16, n=64, tune_steps=16):
@@ -202,6 +232,8 @@ def main():
     ap.add_argument("--adjoint-snapshots", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=200,
                     help="timed steps of the 64-scenario ensemble workload (0 skips it)")
+    ap.add_argument("--large-steps", type=int, default=100,
+                    help="timed forward steps of the 4096x4096 workload (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the whole grid lives in one MI355X's HBM); "
@@ -255,6 +287,8 @@ def main():
         out["adjoint"] = adj
     if args.ensemble_steps > 0 and fp32c:
         out["ensemble"] = time_ensemble(args.ensemble_steps)
+    if args.large_steps > 0 and fp32c:
+        out["large_4096"] = time_large(args.large_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

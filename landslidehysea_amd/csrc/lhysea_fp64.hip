@@ -436,7 +436,14 @@ __global__ void __launch_bounds__(NT) k_fused(Consts k, const double* __restrict
                 a.emax[2 * g + 1] = tnow;
             }
         }
-        cell_update(k, o, ac, dt, n);
+        // a wavefront of cells without sediment and layer-2 sums takes the
+        // reduced form (same values)
+        const bool bare = (o[3] == 0.0) && (o[4] == 0.0) && (o[5] == 0.0) && (ac[3] == 0.0) && (ac[4] == 0.0) &&
+                          (ac[5] == 0.0);
+        if (__all(bare))
+            cell_update1(k, o, ac, dt, n);
+        else
+            cell_update(k, o, ac, dt, n);
 #pragma unroll
         for (int f = 0; f < 6; f++) Wn[f * N + g] = n[f];
     }

@@ -498,5 +498,48 @@ __device__ __forceinline__ void cell_update(const Consts& k, const double* o, co
     n[3] = n20; n[4] = n21; n[5] = n22;
 }
 
+// cell_update() for a cell without sediment: o[3..5] and acc[3..5] are zero.
+// The layer-2 sums stay zero, their damping, speed cap and the Coulomb
+// scaling multiply zeros, and disImplicita can only take its water-bottom
+// case (for either friction law), so those steps are dropped; the layer-1
+// steps are cell_update()'s in its order and give the same values.
+__device__ __forceinline__ void cell_update1(const Consts& k, const double* o, const double* acc, double dt,
+                                             double* n)
+{
+    const double eps = k.eps_h, g = k.g;
+    double val = dt / k.area;
+    double n10 = o[0] + val * acc[0], n11 = o[1] + val * acc[1], n12 = o[2] + val * acc[2];
+
+    // filtroEstado
+    if (n10 < 0.0) n10 = 0.0;
+    double aux0 = 1.0 / (p4(n10 / eps) + LH_EPS);
+    double aux = exp(-dt * aux0);
+    n11 *= aux; n12 *= aux;
+    if (k.vmax1 > 0.0) {
+        double h = n10, hm = sqrt(p4(h) + p4(fmax(h, eps)));
+        double ux = LH_SQRT2 * n11 * h / hm, uy = LH_SQRT2 * n12 * h / hm;
+        double u = sqrt(ux * ux + uy * uy);
+        if (u > k.vmax1) { ux *= k.vmax1 / u; uy *= k.vmax1 / u; n11 = ux * h; n12 = uy * h; }
+    }
+
+    // disImplicita, water-bottom case
+    {
+        double h1 = n10;
+        double h1m = sqrt(p4(h1) + p4(fmax(h1, eps)));
+        double hmod = sqrt(p4(o[0]) + p4(fmax(o[0], eps)));
+        double uo1x = LH_SQRT2 * o[1] * o[0] / hmod, uo1y = LH_SQRT2 * o[2] * o[0] / hmod;
+        double u1 = sqrt(uo1x * uo1x + uo1y * uo1y);
+        if (h1 > 0) {
+            double u1x = LH_SQRT2 * n11 * h1 / h1m;
+            double u1y = LH_SQRT2 * n12 * h1 / h1m;
+            double c1 = dt * g * k.mf0 * k.mf0 * u1 / (pow(h1, 4.0 / 3.0) + LH_EPS);
+            n11 = h1 * u1x / (1.0 + c1);
+            n12 = h1 * u1y / (1.0 + c1);
+        }
+    }
+    n[0] = n10; n[1] = n11; n[2] = n12;
+    n[3] = 0.0; n[4] = 0.0; n[5] = 0.0;
+}
+
 }  // namespace f64
 }  // namespace lhysea

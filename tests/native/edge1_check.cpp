@@ -7,7 +7,8 @@
 //  * cell_update1() must give cell_update()'s layer-1 state and cell_update()
 //    a zero layer-2 state, on random water-only cells with zero layer-2 sums;
 //  * the FP64 edge1() must give the FP64 edge()'s layer-1 fluxes and wave
-//    speed, for the Coulomb and the Pouliquen law.
+//    speed, and the FP64 cell_update1() cell_update()'s state, for the
+//    Coulomb and the Pouliquen law.
 // Build: g++ -std=c++17 -O2 -ffp-contract=off -I tests/native edge1_check.cpp
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -187,5 +188,36 @@ int main(int argc, char** argv)
                         A.Fm[2], B.Fm[0], B.Fm[1], B.Fm[2], A.lam, B.lam);
     }
     std::printf("fp64 edges %ld wet %ld mismatches %ld\n", iters, dwet, dbad);
-    return (bad + cbad + dbad) != 0;
+
+    long dcbad = 0;
+    for (long it = 0; it < iters; it++) {
+        lhysea::Consts k{};
+        k.r = 0.45 + 0.15 * U(rng);
+        k.g = 9.81;
+        k.eps_h = 5e-3;
+        k.area = 3091.0;
+        k.mu = 0.17;
+        k.peso = 1.0;
+        k.beta = 1.0;
+        k.friction_law = U(rng) < 0.3 ? 1 : 0;
+        k.tan1 = 0.1; k.tan2 = 0.3; k.tan3 = 0.12; k.tan4 = 0.35; k.LoverH = 1.0;
+        k.mfc = 0.08; k.mf0 = 0.03 * U(rng); k.mfs = 0.03;
+        k.vmax1 = U(rng) < 0.3 ? 5 * U(rng) : 0.0;
+        k.vmax2 = U(rng) < 0.3 ? 5 * U(rng) : 0.0;
+        const double dt = 0.05 + 0.3 * U(rng);
+        const double q = U(rng);
+        const double h = q < 0.1 ? 0.0 : q < 0.3 ? 0.02 * U(rng) : 300 * U(rng);
+        const double o[6] = {h, (U(rng) - 0.5) * 40 * (h > 0), (U(rng) - 0.5) * 40 * (h > 0), 0.0, 0.0, 0.0};
+        const double acc[6] = {(U(rng) - 0.5) * 2e3, (U(rng) - 0.5) * 2e4, (U(rng) - 0.5) * 2e4, 0.0, 0.0, 0.0};
+        double na[6], nb[6];
+        lhysea::f64::cell_update(k, o, acc, dt, na);
+        lhysea::f64::cell_update1(k, o, acc, dt, nb);
+        bool ok = true;
+        for (int f = 0; f < 3; f++) ok = ok && na[f] == nb[f];
+        for (int f = 3; f < 6; f++) ok = ok && na[f] == 0.0 && nb[f] == 0.0;
+        if (!ok && ++dcbad <= 5)
+            std::printf("fp64 cell mismatch at %ld: %g %g %g / %g %g %g\n", it, na[0], na[1], na[2], nb[0], nb[1], nb[2]);
+    }
+    std::printf("fp64 cells %ld mismatches %ld\n", iters, dcbad);
+    return (bad + cbad + dbad + dcbad) != 0;
 }
