@@ -91,6 +91,11 @@ int lhysea_initial_dt(lhysea_sim *s, double *dt);
 int lhysea_set_dt(lhysea_sim *s, double dt);
 /* Run nsteps steps asynchronously on the handle's stream (no host sync). */
 int lhysea_step(lhysea_sim *s, int nsteps);
+/* Step while t < t_end (the reference's loop condition, src/GPU/ShallowWater.cu:269)
+ * and at most max_steps steps; *steps_done (may be NULL) gets the steps run.
+ * The fused kernels stop on the device, so the host waits once per chunk of
+ * steps, not once per step. */
+int lhysea_run_until(lhysea_sim *s, double t_end, int32_t max_steps, int32_t *steps_done);
 int lhysea_sync(lhysea_sim *s);
 /* time at the current state, dt of the next step, completed steps */
 int lhysea_get_time(lhysea_sim *s, double *t, double *dt_next, int32_t *steps);

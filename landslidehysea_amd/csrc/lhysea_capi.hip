@@ -107,6 +107,7 @@ static StepArgs step_args(lhysea_sim* s)
     a.gcap = s->gcap;
     a.dt_hist = s->dt_hist;
     a.hcap = s->hcap;
+    a.t_end = INFINITY;
     return a;
 }
 
@@ -139,9 +140,11 @@ static void* state_buf(lhysea_sim* s, int b)
     return s->p.precision == LHYSEA_FP64 ? (void*)s->b64.W[b] : (void*)s->b32.W[b];
 }
 
-static int launch_steps(lhysea_sim* s, int nsteps, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+static int launch_steps(lhysea_sim* s, int nsteps, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+                        double t_end = INFINITY)
 {
-    const StepArgs a = launch_args(s, ev0, ev1);
+    StepArgs a = launch_args(s, ev0, ev1);
+    a.t_end = t_end;
     for (int n = 0; n < nsteps; n++) {
         const int nxt = 1 - s->cur;
         hipError_t e = launch_one(s, state_buf(s, s->cur), state_buf(s, nxt), a);
@@ -323,6 +326,40 @@ int lhysea_step(lhysea_sim* s, int nsteps)
         if (rc) return rc;
     }
     return launch_steps(s, nsteps);
+}
+
+int lhysea_run_until(lhysea_sim* s, double t_end, int32_t max_steps, int32_t* steps_done)
+{
+    if (!s || max_steps < 0) return fail(LHYSEA_ERR_ARGUMENT, "bad handle or step budget");
+    Ctl c;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    if (!(c.dt_cur > 0) && !s->dt_seq) {
+        int rc = lhysea_initial_dt(s, nullptr);
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    }
+    const int step0 = c.step;
+    const bool device_stop = (s->mode == MODE_FUSED);
+    while (c.t < t_end && c.step - step0 < max_steps) {
+        // chunks sized from the current dt; steps launched after t reaches
+        // t_end do nothing (fused kernels), so the host syncs once per chunk
+        int chunk = 1;
+        if (device_stop) {
+            const double est = (c.dt_cur > 0) ? (t_end - c.t) / c.dt_cur : 1.0;
+            chunk = (est > 511.0) ? 512 : (int)est + 1;
+        }
+        if (chunk > max_steps - (c.step - step0)) chunk = max_steps - (c.step - step0);
+        const int cur0 = s->cur, stepc = c.step;
+        int rc = launch_steps(s, chunk, nullptr, nullptr, device_stop ? t_end : INFINITY);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(s->stream));
+        HIPCHK(hipMemcpy(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+        // the state is in the buffer the last executed step wrote
+        s->cur = cur0 ^ ((c.step - stepc) & 1);
+    }
+    if (steps_done) *steps_done = c.step - step0;
+    return LHYSEA_OK;
 }
 
 int lhysea_sync(lhysea_sim* s)
@@ -718,15 +755,10 @@ int lhysea_shallow_water(const lhysea_params* p, double* W, const double* H, dou
     rc = lhysea_upload(s, W, H);
     if (!rc) rc = lhysea_track_eta_max(s, 1);
     if (!rc) rc = lhysea_initial_dt(s, nullptr);
-    double t = 0.0;
     int32_t n = 0;
     auto t0 = std::chrono::steady_clock::now();
-    // while (t < t_end): one step per iteration; t is read back every step
-    while (!rc) {
-        rc = lhysea_get_time(s, &t, nullptr, &n);
-        if (rc || !(t < t_end)) break;
-        rc = lhysea_step(s, 1);
-    }
+    // while (t < t_end) (src/GPU/ShallowWater.cu:269), stepped on the device
+    if (!rc) rc = lhysea_run_until(s, t_end, INT32_MAX, &n);
     auto t1 = std::chrono::steady_clock::now();
     if (elapsed_s) *elapsed_s = std::chrono::duration<double>(t1 - t0).count();
     if (steps_done) *steps_done = n;

@@ -118,6 +118,7 @@ __global__ void k_dt0_finalize(Consts k, Ctl* ctl)
 __global__ void k_finalize(Consts k, const float* __restrict__ Wn, const double* __restrict__ H, StepArgs a)
 {
     Ctl* ctl = a.ctl;
+    if (!(ctl->t < a.t_end)) return;
     const size_t N = (size_t)k.nx * k.ny;
     const int s = ctl->step;
     const double dt = step_dt(ctl, a.dt_seq);
@@ -487,6 +488,7 @@ template <int TX, int TY, int NT>
 __global__ void __launch_bounds__(NT) k_fused(Consts k, const float* __restrict__ W, float* __restrict__ Wn,
                                               const double* __restrict__ H, StepArgs a, int tiles_x, int tiles_y)
 {
+    if (!(a.ctl->t < a.t_end)) return;     // the run has reached its end time
     const int nb = tiles_x * tiles_y;
     int b;
     {

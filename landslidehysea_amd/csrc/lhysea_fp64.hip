@@ -125,6 +125,7 @@ __global__ void k_dt0_finalize(Consts k, Ctl* ctl)
 __global__ void k_finalize(Consts k, const double* __restrict__ Wn, const double* __restrict__ H, StepArgs a)
 {
     Ctl* ctl = a.ctl;
+    if (!(ctl->t < a.t_end)) return;
     const size_t N = (size_t)k.nx * k.ny;
     const int s = ctl->step;
     const double dt = step_dt(ctl, a.dt_seq);
@@ -280,6 +281,7 @@ __global__ void __launch_bounds__(NT) k_fused(Consts k, const double* __restrict
     __shared__ double sW[7][RW][CW];
     __shared__ double sA[7][TY][CW];
 
+    if (!(a.ctl->t < a.t_end)) return;     // the run has reached its end time
     const int nx = k.nx, ny = k.ny;
     const size_t N = (size_t)nx * ny;
     // XCD-aware tile order: consecutive workgroups go round-robin over the 8

@@ -30,6 +30,7 @@ struct StepArgs {
     double* dt_hist;        // [hcap]
     int hcap;
     hipEvent_t ev0, ev1;    // when set: recorded around the main step kernel only
+    double t_end;           // fused kernels: a step launched at t >= t_end does nothing
 };
 
 // Device buffers of one simulation in FP64.

@@ -110,6 +110,7 @@ SIGNATURES = {
     "lhysea_initial_dt": (C.c_int, [C.c_void_p, _dp]),
     "lhysea_set_dt": (C.c_int, [C.c_void_p, C.c_double]),
     "lhysea_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "lhysea_run_until": (C.c_int, [C.c_void_p, C.c_double, C.c_int32, _ip]),
     "lhysea_sync": (C.c_int, [C.c_void_p]),
     "lhysea_get_time": (C.c_int, [C.c_void_p, _dp, _dp, _ip]),
     "lhysea_track_eta_max": (C.c_int, [C.c_void_p, C.c_int]),
@@ -235,6 +236,12 @@ class Simulation:
 
     def sync(self):
         _check(self.lib.lhysea_sync(self.h))
+
+    def run_until(self, t_end: float, max_steps: int = 2**31 - 1) -> int:
+        """Step while t < t_end (at most max_steps); returns the steps run."""
+        n = C.c_int32()
+        _check(self.lib.lhysea_run_until(self.h, t_end, max_steps, C.byref(n)))
+        return n.value
 
     def time(self):
         t, d, n = C.c_double(), C.c_double(), C.c_int32()
