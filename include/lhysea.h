@@ -117,18 +117,35 @@ int lhysea_set_kernel(lhysea_sim *s, int32_t mode);
 int lhysea_time_steps(lhysea_sim *s, int32_t nsteps, double *total_ms, double *kernel_ms);
 
 /* Discrete adjoint (no counterpart in the reference: an extension of the
- * step for gradient-based inversion).  Runs nsteps forward steps from the freshly
- * uploaded state with a gauge-misfit loss
- *     L = sum_n sum_g (eta_g^n - obs[n*ng + g])^2,  eta = h1 + h2 - H after step n,
- * then the reverse sweep with binomial checkpointing over nsnap state
- * snapshots in HBM (branches and the dt sequence frozen at the forward
- * choices); with nsnap > nsteps every state is kept in HBM instead (nsteps + 1
- * states, no step re-advanced).  Outputs: the loss, dL/dW0 (6 planes like
- * lhysea_upload; may be NULL), dL/dmu of the Coulomb coefficient
- * mu = |tan(angle1)| L/H and dL/dangle1.
+ * step for gradient-based inversion), as a forward/backward pair.
+ *
+ * lhysea_adjoint_forward runs nsteps forward steps from the freshly uploaded
+ * state, records the dt sequence and the gauge series eta = h1 + h2 - H
+ * after each step (eta: nsteps*ng doubles, row n = after step n+1; may be
+ * NULL) and keeps the checkpoints the reverse sweep needs: nsnap state
+ * snapshots in HBM placed by binomial checkpointing, or, with nsnap > nsteps,
+ * every state (nsteps + 1 states, no step re-advanced).  The handle then
+ * holds the workspace and refuses the calls that step or reconfigure it
+ * (upload, initial_dt, set_dt, step, run_until, set_gauges, track_eta_max,
+ * set_kernel, time_steps) until lhysea_adjoint_backward, which consumes it;
+ * lhysea_download returns the state after step nsteps meanwhile.  A second
+ * forward, or lhysea_destroy, discards a pending workspace.
+ *
+ * lhysea_adjoint_backward takes the seeds dL/deta (nsteps*ng doubles, the
+ * layout of eta) of any loss L of the gauge series and runs the reverse sweep
+ * (branches and the dt sequence frozen at the forward choices).  Outputs:
+ * dL/dW0 (6 planes like lhysea_upload; may be NULL), dL/dmu of the Coulomb
+ * coefficient mu = |tan(angle1)| L/H and dL/dangle1 (may be NULL).
  * stats (6 doubles, may be NULL): forward ms, reverse ms, re-advanced steps,
  * reverse steps, snapshots, optimal re-advance count.  Afterwards the handle
- * must be uploaded again before further steps. */
+ * must be uploaded again before further steps.
+ *
+ * lhysea_adjoint is both for the gauge-misfit loss
+ *     L = sum_n sum_g (eta_g^n - obs[n*ng + g])^2,   seeds 2 (eta - obs). */
+int lhysea_adjoint_forward(lhysea_sim *s, int32_t nsteps, int32_t nsnap, const int32_t *gauges, int32_t ng,
+                           double *eta);
+int lhysea_adjoint_backward(lhysea_sim *s, const double *seeds, double *grad_W0, double *grad_mu,
+                            double *grad_angle1, double *stats);
 int lhysea_adjoint(lhysea_sim *s, int32_t nsteps, int32_t nsnap, const int32_t *gauges, int32_t ng,
                    const double *obs, double *loss, double *grad_W0, double *grad_mu, double *grad_angle1,
                    double *stats);

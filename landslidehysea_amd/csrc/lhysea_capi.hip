@@ -38,6 +38,8 @@ int hip_fail(hipError_t e, const char* where)
 
 void lhysea::set_last_error(const std::string& msg) { g_err = msg; }
 
+struct lhysea_adj_work;
+
 struct lhysea_sim {
     lhysea_params p{};
     Consts k{};
@@ -59,7 +61,16 @@ struct lhysea_sim {
     double* dt_hist = nullptr;
     int hcap = 0;
     double* dt_seq = nullptr;    // frozen dt sequence (adjoint replay)
+    lhysea_adj_work* adj = nullptr;  // adjoint forward pass awaiting its reverse sweep
 };
+
+
+// Calls that step or reconfigure the handle are refused between an adjoint
+// forward pass and its reverse sweep, which replays the recorded steps.
+static int adj_busy()
+{
+    return fail(LHYSEA_ERR_ARGUMENT, "an adjoint forward pass awaits lhysea_adjoint_backward");
+}
 
 Consts lhysea::make_consts(const lhysea_params& p)
 {
@@ -156,6 +167,8 @@ static int launch_steps(lhysea_sim* s, int nsteps, hipEvent_t ev0 = nullptr, hip
 
 extern "C" {
 
+static void adj_release(lhysea_sim* s);
+
 int lhysea_device_probe(void)
 {
     int n = 0;
@@ -223,6 +236,7 @@ int lhysea_create(const lhysea_params* p, lhysea_sim** out)
 int lhysea_destroy(lhysea_sim* s)
 {
     if (!s) return LHYSEA_OK;
+    adj_release(s);
     if (s->stream) hipStreamSynchronize(s->stream);
     for (int b = 0; b < 2; b++) {
         hipFree(s->b64.W[b]);
@@ -246,6 +260,7 @@ int lhysea_destroy(lhysea_sim* s)
 int lhysea_upload(lhysea_sim* s, const double* W, const double* H)
 {
     if (!s || !W || !H) return fail(LHYSEA_ERR_ARGUMENT, "null argument");
+    if (s->adj) return adj_busy();
     const size_t N = s->N;
     HIPCHK(hipStreamSynchronize(s->stream));
     s->cur = 0;
@@ -295,6 +310,7 @@ int lhysea_download(lhysea_sim* s, double* W)
 int lhysea_initial_dt(lhysea_sim* s, double* dt)
 {
     if (!s) return fail(LHYSEA_ERR_ARGUMENT, "null handle");
+    if (s->adj) return adj_busy();
     hipError_t e = (s->p.precision == LHYSEA_FP64) ? f64_initial_dt(s->k, s->b64.W[s->cur], s->ctl, s->stream)
                                                    : f32c_initial_dt(s->k, s->b32.W[s->cur], s->ctl, s->stream);
     if (e != hipSuccess) return hip_fail(e, "initial dt");
@@ -310,6 +326,7 @@ int lhysea_initial_dt(lhysea_sim* s, double* dt)
 int lhysea_set_dt(lhysea_sim* s, double dt)
 {
     if (!s || !(dt > 0)) return fail(LHYSEA_ERR_ARGUMENT, "bad handle or dt");
+    if (s->adj) return adj_busy();
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipMemcpy((char*)s->ctl + offsetof(Ctl, dt_cur), &dt, sizeof(double), hipMemcpyHostToDevice));
     return LHYSEA_OK;
@@ -318,6 +335,7 @@ int lhysea_set_dt(lhysea_sim* s, double dt)
 int lhysea_step(lhysea_sim* s, int nsteps)
 {
     if (!s || nsteps < 0) return fail(LHYSEA_ERR_ARGUMENT, "bad handle or step count");
+    if (s->adj) return adj_busy();
     Ctl c;
     HIPCHK(hipMemcpyAsync(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -331,6 +349,7 @@ int lhysea_step(lhysea_sim* s, int nsteps)
 int lhysea_run_until(lhysea_sim* s, double t_end, int32_t max_steps, int32_t* steps_done)
 {
     if (!s || max_steps < 0) return fail(LHYSEA_ERR_ARGUMENT, "bad handle or step budget");
+    if (s->adj) return adj_busy();
     Ctl c;
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipMemcpy(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
@@ -384,6 +403,7 @@ int lhysea_get_time(lhysea_sim* s, double* t, double* dt_next, int32_t* steps)
 int lhysea_track_eta_max(lhysea_sim* s, int enable)
 {
     if (!s) return fail(LHYSEA_ERR_ARGUMENT, "null handle");
+    if (s->adj) return adj_busy();
     s->track_emax = enable != 0;
     return LHYSEA_OK;
 }
@@ -404,6 +424,7 @@ int lhysea_eta_max(lhysea_sim* s, double* val, double* time)
 int lhysea_set_gauges(lhysea_sim* s, const int32_t* cells, int32_t ng, int32_t max_steps)
 {
     if (!s || ng < 0 || max_steps < 0 || (ng > 0 && !cells)) return fail(LHYSEA_ERR_ARGUMENT, "bad gauge arguments");
+    if (s->adj) return adj_busy();
     for (int g = 0; g < ng; g++)
         if (cells[g] < 0 || (size_t)cells[g] >= s->N) return fail(LHYSEA_ERR_ARGUMENT, "gauge cell out of range");
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -457,6 +478,7 @@ int lhysea_dt_history(lhysea_sim* s, double* out, int32_t max_steps, int32_t* nr
 int lhysea_set_kernel(lhysea_sim* s, int32_t mode)
 {
     if (!s || (mode != MODE_FUSED && mode != MODE_FOURPASS)) return fail(LHYSEA_ERR_ARGUMENT, "bad kernel mode");
+    if (s->adj) return adj_busy();
     if (mode == MODE_FOURPASS && s->p.precision != LHYSEA_FP64)
         return fail(LHYSEA_ERR_ARGUMENT, "the four-pass kernels exist in FP64 only");
     s->mode = mode;
@@ -466,6 +488,7 @@ int lhysea_set_kernel(lhysea_sim* s, int32_t mode)
 int lhysea_time_steps(lhysea_sim* s, int32_t nsteps, double* total_ms, double* kernel_ms)
 {
     if (!s || nsteps < 1) return fail(LHYSEA_ERR_ARGUMENT, "bad arguments");
+    if (s->adj) return adj_busy();
     Ctl c;
     HIPCHK(hipMemcpyAsync(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -510,32 +533,69 @@ int lhysea_time_steps(lhysea_sim* s, int32_t nsteps, double* total_ms, double* k
     return LHYSEA_OK;
 }
 
+// Workspace of one adjoint run, held by the handle from the forward pass to
+// the reverse sweep: the snapshot store, the reverse-step buffers and the
+// loss seeds (all in HBM).
+struct lhysea_adj_work {
+    lhysea_sim* s = nullptr;
+    char* snap = nullptr;  // snapshots of the state (or every state, store_all)
+    size_t state_bytes = 0;
+    void* P = nullptr;     // partial sums [3][2][N]
+    void* ACC = nullptr;   // [6][N]
+    void* AB = nullptr;    // [6][N]
+    double* LAM = nullptr;   // [6][N] state adjoint lambda of the current step
+    double* LAM2 = nullptr;  // [6][N] second lambda buffer of the fused FP32 reverse step
+    double* gmu = nullptr;
+    const char* W = nullptr; // stored state of the reverse step (all states kept), else null
+    int* gauges = nullptr;
+    double* gser = nullptr;  // gauge series of the forward pass
+    double* G = nullptr;     // [nsteps][ng] seeds dL/deta
+    int ng = 0;
+    long nsteps = 0;
+    int nsnap = 0;
+    bool store_all = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    double fwd_ms = 0;       // forward pass time (with the first-descent snapshots)
+    const double* saved_seq = nullptr;
+    bool saved_emax = false;
+};
+
 namespace {
 
-// Context of one adjoint run: the simulation, the snapshot store and the
-// reverse-step workspace (all in HBM).
-struct AdjCtx {
-    lhysea_sim* s;
-    char* snap;            // S snapshots of the state
-    size_t state_bytes;
-    void* P;               // partial sums [3][2][N]
-    void* ACC;             // [6][N]
-    void* AB;              // [6][N]
-    double* LAM;           // [6][N] state adjoint lambda of the current step
-    double* LAM2;          // [6][N] second lambda buffer of the fused FP32 reverse step
-    double* gmu;
-    const char* W;         // stored state of the reverse step (all states kept), else null
-    int* gauges;
-    double* gser;          // gauge series of the forward pass
-    double* G;             // [nsteps][ng] loss gradient wrt eta
-    int ng;
-};
+// Frees a workspace and gives the handle back its stepping configuration.
+// The gauge buffers belong to the adjoint run; the handle's state must be
+// uploaded again before further steps.
+void adj_free(lhysea_sim* s, lhysea_adj_work* a)
+{
+    if (!a) return;
+    hipStreamSynchronize(s->stream);
+    hipFree(a->snap);
+    hipFree(a->P);
+    hipFree(a->ACC);
+    hipFree(a->AB);
+    hipFree(a->LAM);
+    hipFree(a->LAM2);
+    hipFree(a->gmu);
+    hipFree(a->G);
+    if (a->e0) hipEventDestroy(a->e0);
+    if (a->e1) hipEventDestroy(a->e1);
+    if (a->e2) hipEventDestroy(a->e2);
+    s->dt_seq = const_cast<double*>(a->saved_seq);
+    s->track_emax = a->saved_emax;
+    hipFree(a->gser);
+    if (a->gauges) hipFree(a->gauges);
+    if (s->gauges == a->gauges) s->gauges = nullptr;
+    if (s->gseries == a->gser) s->gseries = nullptr;
+    s->ng = 0;
+    s->gcap = 0;
+    delete a;
+}
 
 void* state_ptr(lhysea_sim* s) { return s->p.precision == LHYSEA_FP64 ? (void*)s->b64.W[s->cur] : (void*)s->b32.W[s->cur]; }
 
 hipError_t hook_advance(void* c, int k0, int n)
 {
-    AdjCtx* a = (AdjCtx*)c;
+    lhysea_adj_work* a = (lhysea_adj_work*)c;
     hipError_t e = adj_set_step(a->s->ctl, k0, a->s->stream);
     if (e != hipSuccess) return e;
     return launch_steps(a->s, n) == LHYSEA_OK ? hipSuccess : hipErrorUnknown;
@@ -543,21 +603,21 @@ hipError_t hook_advance(void* c, int k0, int n)
 
 hipError_t hook_store(void* c, int slot)
 {
-    AdjCtx* a = (AdjCtx*)c;
+    lhysea_adj_work* a = (lhysea_adj_work*)c;
     return hipMemcpyAsync(a->snap + (size_t)slot * a->state_bytes, state_ptr(a->s), a->state_bytes,
                           hipMemcpyDeviceToDevice, a->s->stream);
 }
 
 hipError_t hook_restore(void* c, int slot)
 {
-    AdjCtx* a = (AdjCtx*)c;
+    lhysea_adj_work* a = (lhysea_adj_work*)c;
     return hipMemcpyAsync(state_ptr(a->s), a->snap + (size_t)slot * a->state_bytes, a->state_bytes,
                           hipMemcpyDeviceToDevice, a->s->stream);
 }
 
 hipError_t hook_reverse(void* c, int k)
 {
-    AdjCtx* a = (AdjCtx*)c;
+    lhysea_adj_work* a = (lhysea_adj_work*)c;
     lhysea_sim* s = a->s;
     hipError_t e;
     // the state W^k: the working buffer, or its stored slot
@@ -578,154 +638,159 @@ hipError_t hook_reverse(void* c, int k)
 
 }  // namespace
 
-int lhysea_adjoint(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const int32_t* gauges, int32_t ng,
-                   const double* obs, double* loss, double* grad_W0, double* grad_mu, double* grad_angle1,
-                   double* stats)
+static void adj_release(lhysea_sim* s)
 {
-    if (!s || nsteps < 1 || nsnap < 1 || ng < 0 || (ng > 0 && (!gauges || !obs)))
+    lhysea_adj_work* a = s->adj;
+    s->adj = nullptr;
+    adj_free(s, a);
+}
+
+#define ADJCHK(expr)                                          \
+    do {                                                      \
+        hipError_t e_ = (expr);                               \
+        if (e_ != hipSuccess) return done(hip_fail(e_, #expr)); \
+    } while (0)
+
+int lhysea_adjoint_forward(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const int32_t* gauges, int32_t ng,
+                           double* eta)
+{
+    if (!s || nsteps < 1 || nsnap < 1 || ng < 0 || (ng > 0 && !gauges))
         return fail(LHYSEA_ERR_ARGUMENT, "bad adjoint arguments");
     if (nsteps > s->hcap) return fail(LHYSEA_ERR_ARGUMENT, "too many steps for the dt history");
     if (s->p.friction_law != 0) return fail(LHYSEA_ERR_ARGUMENT, "the adjoint implements the Coulomb law");
     for (int g = 0; g < ng; g++)
         if (gauges[g] < 0 || (size_t)gauges[g] >= s->N) return fail(LHYSEA_ERR_ARGUMENT, "gauge cell out of range");
+    // a forward pass whose reverse sweep never came is discarded
+    adj_release(s);
     const size_t N = s->N;
     const bool f64 = s->p.precision == LHYSEA_FP64;
     const size_t fsz = f64 ? sizeof(double) : sizeof(float);
-    AdjCtx a{};
-    a.s = s;
-    a.state_bytes = 48 * N;       // 6 doubles or 12 floats per cell
-    a.ng = ng;
-    int rc = LHYSEA_OK;
-    hipError_t he0 = hipSuccess;
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
-    const double* saved_seq = s->dt_seq;
-    const bool saved_emax = s->track_emax;
+    lhysea_adj_work* a = new (std::nothrow) lhysea_adj_work{};
+    if (!a) return fail(LHYSEA_ERR_HOST_MEMORY, "adjoint workspace");
+    a->s = s;
+    a->state_bytes = 48 * N;       // 6 doubles or 12 floats per cell
+    a->ng = ng;
+    a->nsteps = nsteps;
+    a->nsnap = nsnap;
+    a->saved_seq = s->dt_seq;
+    a->saved_emax = s->track_emax;
     auto done = [&](int code) {
-        hipStreamSynchronize(s->stream);
-        hipFree(a.snap);
-        hipFree(a.P);
-        hipFree(a.ACC);
-        hipFree(a.AB);
-        hipFree(a.LAM);
-        hipFree(a.LAM2);
-        hipFree(a.gmu);
-        hipFree(a.G);
-        if (e0) hipEventDestroy(e0);
-        if (e1) hipEventDestroy(e1);
-        if (e2) hipEventDestroy(e2);
-        s->dt_seq = const_cast<double*>(saved_seq);
-        s->track_emax = saved_emax;
-        // the gauge buffers belong to this run; the handle's state must be
-        // uploaded again before further steps
-        hipFree(a.gser);
-        if (a.gauges) hipFree(a.gauges);
-        if (s->gauges == a.gauges) s->gauges = nullptr;
-        if (s->gseries == a.gser) s->gseries = nullptr;
-        s->ng = 0;
-        s->gcap = 0;
+        adj_free(s, a);
         return code;
     };
-    const size_t nslots = (nsnap > nsteps) ? (size_t)nsteps + 1 : (size_t)nsnap;
-    if (hipMalloc(&a.snap, nslots * a.state_bytes) != hipSuccess || hipMalloc(&a.P, 6 * N * fsz) != hipSuccess ||
-        hipMalloc(&a.ACC, 6 * N * fsz) != hipSuccess || hipMalloc(&a.AB, 6 * N * fsz) != hipSuccess ||
-        hipMalloc(&a.LAM, 6 * N * sizeof(double)) != hipSuccess ||
-        (!f64 && hipMalloc(&a.LAM2, 6 * N * sizeof(double)) != hipSuccess) ||
-        hipMalloc(&a.gmu, sizeof(double)) != hipSuccess ||
-        hipMalloc(&a.G, (size_t)nsteps * (ng > 0 ? ng : 1) * sizeof(double)) != hipSuccess)
+    // with a snapshot per state (nsnap > nsteps) every state stays in HBM:
+    // the forward writes step k's result straight into slot k + 1 and the
+    // reverse sweep reads slot k in place (no re-advance, no copy)
+    a->store_all = nsnap > nsteps;
+    const size_t nslots = a->store_all ? (size_t)nsteps + 1 : (size_t)nsnap;
+    if (hipMalloc(&a->snap, nslots * a->state_bytes) != hipSuccess || hipMalloc(&a->P, 6 * N * fsz) != hipSuccess ||
+        hipMalloc(&a->ACC, 6 * N * fsz) != hipSuccess || hipMalloc(&a->AB, 6 * N * fsz) != hipSuccess ||
+        hipMalloc(&a->LAM, 6 * N * sizeof(double)) != hipSuccess ||
+        (!f64 && hipMalloc(&a->LAM2, 6 * N * sizeof(double)) != hipSuccess) ||
+        hipMalloc(&a->gmu, sizeof(double)) != hipSuccess ||
+        hipMalloc(&a->G, (size_t)nsteps * (ng > 0 ? ng : 1) * sizeof(double)) != hipSuccess)
         return done(fail(LHYSEA_ERR_GPU_MEMORY, "adjoint workspace allocation failed"));
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventCreate(&e2);
+    ADJCHK(hipEventCreate(&a->e0));
+    ADJCHK(hipEventCreate(&a->e1));
+    ADJCHK(hipEventCreate(&a->e2));
 
-    // 1. forward pass from the uploaded state: dt history, gauge series and the
-    //    snapshots of the first descent of the checkpoint tree
-    rc = lhysea_set_gauges(s, gauges, ng, nsteps);
+    // forward pass from the uploaded state: dt history, gauge series and the
+    // snapshots of the first descent of the checkpoint tree
+    int rc = lhysea_set_gauges(s, gauges, ng, nsteps);
     if (rc) return done(rc);
-    a.gauges = s->gauges;
-    a.gser = s->gseries;
+    a->gauges = s->gauges;
+    a->gser = s->gseries;
     s->track_emax = false;
     s->dt_seq = nullptr;
     Ctl c0;
-    HIPCHK(hipMemcpy(&c0, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    ADJCHK(hipMemcpy(&c0, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
     if (c0.step != 0) return done(fail(LHYSEA_ERR_ARGUMENT, "the adjoint starts from a freshly uploaded state"));
     if (!(c0.dt_cur > 0)) {
         rc = lhysea_initial_dt(s, nullptr);
         if (rc) return done(rc);
     }
-    // with a snapshot per state (nsnap > nsteps) every state stays in HBM:
-    // the forward writes step k's result straight into slot k + 1 and the
-    // reverse sweep reads slot k in place (no re-advance, no copy)
-    const bool store_all = nsnap > nsteps;
-    std::vector<std::pair<long, int>> chain;
-    if (!store_all) revolve_chain(nsteps, nsnap, chain);
-    hipEventRecord(e0, s->stream);
-    if (hook_store(&a, 0) != hipSuccess) return done(fail(LHYSEA_ERR_HIP, "snapshot copy failed"));
-    if (store_all) {
+    ADJCHK(hipEventRecord(a->e0, s->stream));
+    ADJCHK(hook_store(a, 0));
+    if (a->store_all) {
         const StepArgs sa = launch_args(s, nullptr, nullptr);
-        for (long k = 0; k < nsteps; k++) {
-            he0 = launch_one(s, a.snap + (size_t)k * a.state_bytes, a.snap + (size_t)(k + 1) * a.state_bytes, sa);
-            if (he0 != hipSuccess) return done(hip_fail(he0, "step launch"));
-        }
+        for (long k = 0; k < nsteps; k++)
+            ADJCHK(launch_one(s, a->snap + (size_t)k * a->state_bytes, a->snap + (size_t)(k + 1) * a->state_bytes, sa));
     } else {
+        std::vector<std::pair<long, int>> chain;
+        revolve_chain(nsteps, nsnap, chain);
         long pos = 0;
         for (auto& ch : chain) {
             rc = launch_steps(s, (int)(ch.first - pos));
             if (rc) return done(rc);
-            if (hook_store(&a, ch.second) != hipSuccess) return done(fail(LHYSEA_ERR_HIP, "snapshot copy failed"));
+            ADJCHK(hook_store(a, ch.second));
             pos = ch.first;
         }
         rc = launch_steps(s, (int)(nsteps - pos));
         if (rc) return done(rc);
     }
-    hipEventRecord(e1, s->stream);
-
-    // 2. loss and its gradient with respect to the gauge series
-    std::vector<double> gs((size_t)nsteps * ng), G((size_t)nsteps * ng);
-    int32_t nrec = 0;
-    rc = lhysea_get_gauges(s, gs.data(), &nrec);
-    if (rc) return done(rc);
-    double L = 0.0;
-    for (size_t i = 0; i < gs.size(); i++) {
-        const double d = gs[i] - obs[i];
-        L += d * d;
-        G[i] = 2.0 * d;
+    ADJCHK(hipEventRecord(a->e1, s->stream));
+    ADJCHK(hipStreamSynchronize(s->stream));
+    ADJCHK(hipGetLastError());
+    float f_ms = 0;
+    hipEventElapsedTime(&f_ms, a->e0, a->e1);
+    a->fwd_ms = f_ms;
+    if (eta && ng > 0) {
+        int32_t nrec = 0;
+        rc = lhysea_get_gauges(s, eta, &nrec);
+        if (rc) return done(rc);
     }
-    if (loss) *loss = L;
-    if (ng > 0) HIPCHK(hipMemcpy(a.G, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice));
     // gauges off for the re-advances; the dt history is frozen from here on
     s->gseries = nullptr;
     s->ng = 0;
     s->dt_seq = s->dt_hist;
-    HIPCHK(hipMemsetAsync(a.LAM, 0, 6 * N * sizeof(double), s->stream));
-    HIPCHK(hipMemsetAsync(a.gmu, 0, sizeof(double), s->stream));
-    if (hook_restore(&a, 0) != hipSuccess) return done(fail(LHYSEA_ERR_HIP, "snapshot copy failed"));
-    if (adj_inject(a.LAM, N, a.gauges, ng, a.G + (size_t)(nsteps - 1) * ng, s->stream) != hipSuccess)
-        return done(fail(LHYSEA_ERR_HIP, "loss injection failed"));
+    s->adj = a;
+    return LHYSEA_OK;
+}
 
-    // 3. reverse sweep with binomial checkpointing (or over the stored states)
-    AdjointHooks h{hook_advance, hook_store, hook_restore, hook_reverse, &a, 0, 0};
+int lhysea_adjoint_backward(lhysea_sim* s, const double* seeds, double* grad_W0, double* grad_mu,
+                            double* grad_angle1, double* stats)
+{
+    if (!s) return fail(LHYSEA_ERR_ARGUMENT, "null handle");
+    if (!s->adj) return fail(LHYSEA_ERR_ARGUMENT, "no adjoint forward pass to reverse");
+    if (s->adj->ng > 0 && !seeds) return fail(LHYSEA_ERR_ARGUMENT, "missing seeds");
+    // the workspace goes whatever happens below
+    lhysea_adj_work* a = s->adj;
+    s->adj = nullptr;
+    auto done = [&](int code) {
+        adj_free(s, a);
+        return code;
+    };
+    const size_t N = s->N;
+    const long nsteps = a->nsteps;
+    const int ng = a->ng;
+    if (ng > 0) ADJCHK(hipMemcpy(a->G, seeds, (size_t)nsteps * ng * sizeof(double), hipMemcpyHostToDevice));
+    ADJCHK(hipEventRecord(a->e1, s->stream));
+    ADJCHK(hipMemsetAsync(a->LAM, 0, 6 * N * sizeof(double), s->stream));
+    ADJCHK(hipMemsetAsync(a->gmu, 0, sizeof(double), s->stream));
+    ADJCHK(hook_restore(a, 0));
+    ADJCHK(adj_inject(a->LAM, N, a->gauges, ng, a->G + (size_t)(nsteps - 1) * ng, s->stream));
+
+    // reverse sweep with binomial checkpointing (or over the stored states)
+    AdjointHooks h{hook_advance, hook_store, hook_restore, hook_reverse, a, 0, 0};
     hipError_t he = hipSuccess;
-    if (store_all) {
+    if (a->store_all) {
         for (long k = nsteps - 1; k >= 0 && he == hipSuccess; k--) {
-            a.W = a.snap + (size_t)k * a.state_bytes;
-            he = hook_reverse(&a, (int)k);
+            a->W = a->snap + (size_t)k * a->state_bytes;
+            he = hook_reverse(a, (int)k);
             h.reverses++;
         }
-        a.W = nullptr;
+        a->W = nullptr;
     } else {
-        he = run_treeverse(h, nsnap, nsteps);
+        he = run_treeverse(h, a->nsnap, nsteps);
     }
     if (he != hipSuccess) return done(hip_fail(he, "adjoint sweep"));
-    hipEventRecord(e2, s->stream);
-    HIPCHK(hipStreamSynchronize(s->stream));
-    he = hipGetLastError();
-    if (he != hipSuccess) return done(hip_fail(he, "adjoint sweep"));
+    ADJCHK(hipEventRecord(a->e2, s->stream));
+    ADJCHK(hipStreamSynchronize(s->stream));
+    ADJCHK(hipGetLastError());
 
-    std::vector<double> lam(6 * N);
     double gm = 0.0;
-    HIPCHK(hipMemcpy(lam.data(), a.LAM, 6 * N * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(&gm, a.gmu, sizeof(double), hipMemcpyDeviceToHost));
-    if (grad_W0) std::copy(lam.begin(), lam.end(), grad_W0);
+    if (grad_W0) ADJCHK(hipMemcpy(grad_W0, a->LAM, 6 * N * sizeof(double), hipMemcpyDeviceToHost));
+    ADJCHK(hipMemcpy(&gm, a->gmu, sizeof(double), hipMemcpyDeviceToHost));
     if (grad_mu) *grad_mu = gm;
     if (grad_angle1) {
         // mu = |tan a| L/H  ->  dmu/da = sign(tan a) (1 + tan^2 a) L/H
@@ -733,17 +798,36 @@ int lhysea_adjoint(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const int32_t* 
         *grad_angle1 = gm * (t >= 0 ? 1.0 : -1.0) * (1.0 + t * t) * (s->p.L / s->p.H);
     }
     if (stats) {
-        float f_ms = 0, r_ms = 0;
-        hipEventElapsedTime(&f_ms, e0, e1);
-        hipEventElapsedTime(&r_ms, e1, e2);
-        stats[0] = f_ms;                       // forward pass (with first-descent snapshots)
+        float r_ms = 0;
+        hipEventElapsedTime(&r_ms, a->e1, a->e2);
+        stats[0] = a->fwd_ms;                  // forward pass (with first-descent snapshots)
         stats[1] = r_ms;                       // reverse sweep (re-advances + reverse steps)
         stats[2] = (double)h.advances;         // re-advanced forward steps
         stats[3] = (double)h.reverses;         // reverse steps
-        stats[4] = (double)nsnap;
-        stats[5] = (double)revolve_advances(nsteps, nsnap);
+        stats[4] = (double)a->nsnap;
+        stats[5] = (double)revolve_advances(nsteps, a->nsnap);
     }
     return done(LHYSEA_OK);
+}
+
+int lhysea_adjoint(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const int32_t* gauges, int32_t ng,
+                   const double* obs, double* loss, double* grad_W0, double* grad_mu, double* grad_angle1,
+                   double* stats)
+{
+    if (!s || nsteps < 1 || ng < 0 || (ng > 0 && (!gauges || !obs)))
+        return fail(LHYSEA_ERR_ARGUMENT, "bad adjoint arguments");
+    // gauge misfit L = sum (eta - obs)^2 and its seeds dL/deta = 2 (eta - obs)
+    std::vector<double> G((size_t)nsteps * ng);
+    int rc = lhysea_adjoint_forward(s, nsteps, nsnap, gauges, ng, G.data());
+    if (rc) return rc;
+    double L = 0.0;
+    for (size_t i = 0; i < G.size(); i++) {
+        const double d = G[i] - obs[i];
+        L += d * d;
+        G[i] = 2.0 * d;
+    }
+    if (loss) *loss = L;
+    return lhysea_adjoint_backward(s, G.data(), grad_W0, grad_mu, grad_angle1, stats);
 }
 
 int lhysea_shallow_water(const lhysea_params* p, double* W, const double* H, double t_end, double* eta_max,

@@ -120,6 +120,8 @@ SIGNATURES = {
     "lhysea_dt_history": (C.c_int, [C.c_void_p, _dp, C.c_int32, _ip]),
     "lhysea_set_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
     "lhysea_time_steps": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
+    "lhysea_adjoint_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, C.c_int32, _dp]),
+    "lhysea_adjoint_backward": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
     "lhysea_adjoint": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, C.c_int32, _dp, _dp, _dp, _dp, _dp,
                                  _dp]),
     "lhysea_shallow_water": (C.c_int, [C.POINTER(CParams), _dp, _dp, C.c_double, _dp, _dp, _dp, _ip]),
@@ -280,6 +282,33 @@ class Simulation:
         _check(self.lib.lhysea_time_steps(self.h, nsteps, C.byref(tot), C.byref(ker) if per_kernel else None))
         return tot.value, (ker.value if per_kernel else None)
 
+
+    def adjoint_forward(self, nsteps: int, nsnap: int, gauges) -> np.ndarray:
+        """Forward pass of the adjoint pair from the uploaded state: returns the
+        gauge series eta (nsteps, ng) and keeps the checkpoints in the handle
+        until adjoint_backward."""
+        g = np.ascontiguousarray(gauges, dtype=np.int32)
+        eta = np.zeros((nsteps, len(g)))
+        _check(self.lib.lhysea_adjoint_forward(self.h, nsteps, nsnap, g.ctypes.data_as(_ip), len(g), _dptr(eta)))
+        self._adj = (nsteps, len(g))
+        return eta
+
+    def adjoint_backward(self, seeds):
+        """Reverse sweep for the seeds dL/deta (nsteps, ng) of any loss of the
+        gauge series: dL/dW0 (6, ny, nx), dL/dmu, dL/dangle1 (radians)."""
+        if not getattr(self, "_adj", None):
+            raise LhyseaError(3, "no adjoint forward pass to reverse")
+        nsteps, ng = self._adj
+        seeds = np.ascontiguousarray(seeds, dtype=np.float64).reshape(nsteps, ng)
+        gW = np.zeros((6, self.params.ny, self.params.nx))
+        gmu, gang = C.c_double(), C.c_double()
+        stats = np.zeros(6)
+        _check(self.lib.lhysea_adjoint_backward(self.h, _dptr(seeds), _dptr(gW), C.byref(gmu), C.byref(gang),
+                                                _dptr(stats)))
+        self._adj = None
+        return {"grad_W0": gW, "grad_mu": gmu.value, "grad_angle1": gang.value,
+                "forward_ms": stats[0], "reverse_ms": stats[1], "advances": int(stats[2]),
+                "reverse_steps": int(stats[3]), "snapshots": int(stats[4]), "optimal_advances": int(stats[5])}
 
     def adjoint(self, nsteps: int, nsnap: int, gauges, obs):
         """Gauge-misfit loss over nsteps steps from the uploaded state and its

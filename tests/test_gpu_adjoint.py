@@ -177,3 +177,127 @@ def test_all_states_kept_equals_checkpointing():
     assert b["advances"] == 0 and b["reverse_steps"] == N
     assert a["loss"] == b["loss"]
     assert np.array_equal(a["grad_W0"], b["grad_W0"]) and a["grad_mu"] == b["grad_mu"]
+
+
+def test_split_pair_equals_misfit_call():
+    """lhysea_adjoint_forward + seeds 2 (eta - obs) + lhysea_adjoint_backward is
+    the one-call misfit adjoint, bit for bit; the forward's gauge series is the
+    plain stepping loop's."""
+    N = 30
+    for precision in (L.FP64, L.FP32C):
+        sc, W, H, prm = case(40, 30, angle=2.0, semi_axes=(8.0, 6.0), precision=precision)
+        gauges = gauge_cells(sc, 5, seed=4)
+        rng = np.random.default_rng(11)
+        obs = 1e-3 * rng.standard_normal((N, len(gauges)))
+        say(f"split adjoint precision={precision} 40x30 N={N}")
+        with L.Simulation(prm) as sim:
+            sim.upload(W, H)
+            sim.initial_dt()
+            sim.set_gauges(gauges, N)
+            sim.step(N)
+            plain = sim.gauges()
+        with L.Simulation(prm) as sim:
+            sim.upload(W, H)
+            one = sim.adjoint(N, 7, gauges, obs)
+        with L.Simulation(prm) as sim:
+            sim.upload(W, H)
+            eta = sim.adjoint_forward(N, 7, gauges)
+            two = sim.adjoint_backward(2.0 * (eta - obs))
+        assert np.array_equal(eta, plain)
+        assert float(((eta - obs) ** 2).sum()) == pytest.approx(one["loss"], rel=1e-15)
+        assert np.array_equal(one["grad_W0"], two["grad_W0"])
+        assert one["grad_mu"] == two["grad_mu"] and one["grad_angle1"] == two["grad_angle1"]
+        assert one["advances"] == two["advances"] and two["reverse_steps"] == N
+
+
+def test_pending_forward_refuses_steps():
+    """Between the pair's calls the handle refuses stepping, keeps the state
+    after the last step for download, and a second forward replaces the first."""
+    N = 12
+    sc, W, H, prm = case(24, 18, angle=2.0, semi_axes=(5.0, 4.0), precision=L.FP32C)
+    gauges = gauge_cells(sc, 3, seed=1)
+    with L.Simulation(prm) as sim:
+        sim.upload(W, H)
+        sim.initial_dt()
+        sim.step(N)
+        after = sim.download()
+    with L.Simulation(prm) as sim:
+        with pytest.raises(L.LhyseaError):
+            sim.adjoint_backward(np.zeros((N, len(gauges))))
+        sim.upload(W, H)
+        sim.adjoint_forward(N, 4, gauges)
+        for call in (lambda: sim.step(1), lambda: sim.run_until(1e9, 1), lambda: sim.upload(W, H),
+                     lambda: sim.set_dt(0.1), lambda: sim.set_gauges(gauges, 4)):
+            with pytest.raises(L.LhyseaError):
+                call()
+        assert np.array_equal(sim.download(), after)
+        first = sim.adjoint_backward(np.ones((N, len(gauges))))
+        with pytest.raises(L.LhyseaError):
+            sim.adjoint_backward(np.ones((N, len(gauges))))
+    with L.Simulation(prm) as sim:
+        sim.upload(W, H)
+        sim.adjoint_forward(N, 4, gauges)
+        # a second forward discards the pending one, then finds the state after
+        # N steps, not a fresh upload; the handle is free again afterwards
+        with pytest.raises(L.LhyseaError):
+            sim.adjoint_forward(N, 4, gauges)
+        sim.upload(W, H)
+        sim.adjoint_forward(N, 4, gauges)
+        again = sim.adjoint_backward(np.ones((N, len(gauges))))
+        assert np.array_equal(again["grad_W0"], first["grad_W0"])
+    with L.Simulation(prm) as sim:
+        sim.upload(W, H)
+        sim.adjoint_forward(N, 4, gauges)
+    # destroy with a pending forward frees its workspace (no leak, no error)
+    assert np.isfinite(first["grad_W0"]).all()
+
+
+def test_series_autograd_any_loss_vs_oracle_fd(oracle):
+    """GaugeSeries: the torch loss L = sum(w * eta) + 0.5 sum(eta_last^2), not the
+    misfit, differentiated by the C-ABI pair vs central differences of the FP64
+    oracle (dt frozen), bar 1e-4 relative on smooth cells and on angle1."""
+    from landslidehysea_amd.autograd import GaugeSeries
+    N = 30
+    sc, W, H, prm, po, gauges, obs, dts = setup(oracle, L.FP64, N=N)
+    w = np.random.default_rng(21).standard_normal((N, len(gauges)))
+
+    def loss_np(series):
+        return float((w * series).sum() + 0.5 * (series[-1] ** 2).sum())
+
+    spec = MisfitSpec(prm, W, H, gauges, None, N, 6)
+    h2 = torch.tensor(W[3], dtype=torch.float64, requires_grad=True)
+    ang = torch.tensor(math.radians(2.0), dtype=torch.float64, requires_grad=True)
+    eta = GaugeSeries.apply(h2, ang, spec)
+    loss = (torch.from_numpy(w) * eta).sum() + 0.5 * (eta[-1] ** 2).sum()
+    loss.backward()
+    L0 = loss_np(oracle.run_fixed_dt(po, W.copy(), H, dts, gauges))
+    say("series loss", loss.item(), "oracle", L0)
+    assert abs(loss.item() - L0) <= 1e-9 * abs(L0)
+    rng = np.random.default_rng(13)
+    checked = 0
+    for c in rng.choice(np.flatnonzero(W[3].ravel() > 5.0), 6, replace=False):
+        j, i = divmod(int(c), prm.nx)
+        eps = 1e-5 * W[3, j, i]
+        Wp, Wm = W.copy(), W.copy()
+        Wp[3, j, i] += eps
+        Wm[3, j, i] -= eps
+        lp = loss_np(oracle.run_fixed_dt(po, Wp, H, dts, gauges))
+        lm = loss_np(oracle.run_fixed_dt(po, Wm, H, dts, gauges))
+        fp, fm, fc = (lp - L0) / eps, (L0 - lm) / eps, (lp - lm) / (2 * eps)
+        ad = float(h2.grad[j, i])
+        say(f"cell {c}: autograd {ad:.6e} central {fc:.6e} one-sided {fp:.6e} {fm:.6e}")
+        if abs(fp - fm) <= 1e-3 * abs(fc) + 1e-12:
+            checked += 1
+            assert abs(ad - fc) <= 1e-4 * abs(fc) + 1e-12
+    assert checked >= 3
+    da = 1e-6
+    pp, pm = oracle_params(oracle, prm), oracle_params(oracle, prm)
+    for p, s in ((pp, 1), (pm, -1)):
+        p.angle1 += s * da
+        p.angle2 += s * da
+        p.angle3 += s * da
+        p.angle4 += s * da
+    fd = (loss_np(oracle.run_fixed_dt(pp, W.copy(), H, dts, gauges))
+          - loss_np(oracle.run_fixed_dt(pm, W.copy(), H, dts, gauges))) / (2 * da)
+    say(f"angle: autograd {float(ang.grad):.6e} fd {fd:.6e}")
+    assert abs(float(ang.grad) - fd) <= 1e-4 * abs(fd)
