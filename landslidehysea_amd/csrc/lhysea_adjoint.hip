@@ -47,7 +47,7 @@ struct Work {
     F* ACC;        // [6][N] flux sums
     F* AB;         // [6][N] adjoints of the flux sums
     double* LAM;   // [6][N] state adjoint (h1,q1x,q1y,h2,q2x,q2y)
-    double* gmu;   // scalar d/dmu
+    double* gmu;   // d/dmu, one slot per workgroup
 };
 
 // Edge geometry of edge e in class cls (the reference's class layout,
@@ -233,8 +233,10 @@ __global__ void __launch_bounds__(256) k_cell_adj(Consts k, SV<F> sv, Work<F> w,
         }
         gm = (double)gmu;
     }
+    // one d/dmu slot per workgroup, summed in slot order at the end of the
+    // sweep: the result does not depend on the workgroups' timing
     const double s = block_sum256(gm);
-    if (threadIdx.x == 0 && s != 0.0) atomicAdd(w.gmu, s);
+    if (threadIdx.x == 0 && s != 0.0) w.gmu[blockIdx.x] += s;
 }
 
 // 3. edge adjoints of one class
@@ -594,7 +596,7 @@ __global__ void __launch_bounds__(NT) k_adj_fused(Consts k, const float* __restr
     if (tid == 0) {
         double s = 0.0;
         for (int w = 0; w < NT / 64; w++) s += red[w];
-        if (s != 0.0) atomicAdd(gmu, s);
+        if (s != 0.0) gmu[blockIdx.x] += s;   // this workgroup's d/dmu slot
     }
 }
 
@@ -687,6 +689,9 @@ void revolve_chain(long N, int S, std::vector<std::pair<long, int>>& out)
 
 hipError_t run_treeverse(AdjointHooks& h, int S, long N) { return treeverse(h, S, 0, N, S, true); }
 
+// tile of the fused FP32 reverse step
+constexpr int kAdjTX = 60, kAdjTY = 12;
+
 // launch wrappers for one reverse step (four-pass structure)
 template <typename F>
 hipError_t reverse_step_t(const Consts& k, const F* W, const double* H, F* P, F* ACC, F* AB, double* LAM, double* gmu,
@@ -726,7 +731,7 @@ hipError_t adj_reverse_step_f32_fused(const Consts& k, const float* W, const dou
                                       const double* LAMin, double* LAMout, double* gmu, const double* dt_hist,
                                       int step, hipStream_t s)
 {
-    constexpr int TX = 60, TY = 12, NT = 512;
+    constexpr int TX = kAdjTX, TY = kAdjTY, NT = 512;
     hipError_t e = f32c_recompute(k, W, H, dt_hist, step, P, ACC, s);
     if (e != hipSuccess) return e;
     const int tiles_x = (k.nx + TX - 1) / TX, tiles_y = (k.ny + TY - 1) / TY;
@@ -739,6 +744,14 @@ hipError_t adj_reverse_step_f64(const Consts& k, const double* W, const double* 
                                 double* LAM, double* gmu, const double* dt_hist, int step, hipStream_t s)
 {
     return reverse_step_t<double>(k, W, H, P, ACC, AB, LAM, gmu, dt_hist, step, s);
+}
+
+size_t adj_gmu_slots(const Consts& k)
+{
+    const size_t N = (size_t)k.nx * k.ny;
+    const size_t cells = (N + 255) / 256;
+    const size_t tiles = (size_t)((k.nx + kAdjTX - 1) / kAdjTX) * ((k.ny + kAdjTY - 1) / kAdjTY);
+    return cells > tiles ? cells : tiles;
 }
 
 hipError_t adj_inject(double* LAM, size_t N, const int* gauges, int ng, const double* G, hipStream_t s)

@@ -545,7 +545,8 @@ struct lhysea_adj_work {
     void* AB = nullptr;    // [6][N]
     double* LAM = nullptr;   // [6][N] state adjoint lambda of the current step
     double* LAM2 = nullptr;  // [6][N] second lambda buffer of the fused FP32 reverse step
-    double* gmu = nullptr;
+    double* gmu = nullptr;   // d/dmu slots, one per workgroup
+    size_t ngmu = 0;
     const char* W = nullptr; // stored state of the reverse step (all states kept), else null
     int* gauges = nullptr;
     double* gser = nullptr;  // gauge series of the forward pass
@@ -683,11 +684,12 @@ int lhysea_adjoint_forward(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const i
     // reverse sweep reads slot k in place (no re-advance, no copy)
     a->store_all = nsnap > nsteps;
     const size_t nslots = a->store_all ? (size_t)nsteps + 1 : (size_t)nsnap;
+    a->ngmu = adj_gmu_slots(s->k);
     if (hipMalloc(&a->snap, nslots * a->state_bytes) != hipSuccess || hipMalloc(&a->P, 6 * N * fsz) != hipSuccess ||
         hipMalloc(&a->ACC, 6 * N * fsz) != hipSuccess || hipMalloc(&a->AB, 6 * N * fsz) != hipSuccess ||
         hipMalloc(&a->LAM, 6 * N * sizeof(double)) != hipSuccess ||
         (!f64 && hipMalloc(&a->LAM2, 6 * N * sizeof(double)) != hipSuccess) ||
-        hipMalloc(&a->gmu, sizeof(double)) != hipSuccess ||
+        hipMalloc(&a->gmu, a->ngmu * sizeof(double)) != hipSuccess ||
         hipMalloc(&a->G, (size_t)nsteps * (ng > 0 ? ng : 1) * sizeof(double)) != hipSuccess)
         return done(fail(LHYSEA_ERR_GPU_MEMORY, "adjoint workspace allocation failed"));
     ADJCHK(hipEventCreate(&a->e0));
@@ -766,7 +768,7 @@ int lhysea_adjoint_backward(lhysea_sim* s, const double* seeds, double* grad_W0,
     if (ng > 0) ADJCHK(hipMemcpy(a->G, seeds, (size_t)nsteps * ng * sizeof(double), hipMemcpyHostToDevice));
     ADJCHK(hipEventRecord(a->e1, s->stream));
     ADJCHK(hipMemsetAsync(a->LAM, 0, 6 * N * sizeof(double), s->stream));
-    ADJCHK(hipMemsetAsync(a->gmu, 0, sizeof(double), s->stream));
+    ADJCHK(hipMemsetAsync(a->gmu, 0, a->ngmu * sizeof(double), s->stream));
     ADJCHK(hook_restore(a, 0));
     ADJCHK(adj_inject(a->LAM, N, a->gauges, ng, a->G + (size_t)(nsteps - 1) * ng, s->stream));
 
@@ -788,9 +790,11 @@ int lhysea_adjoint_backward(lhysea_sim* s, const double* seeds, double* grad_W0,
     ADJCHK(hipStreamSynchronize(s->stream));
     ADJCHK(hipGetLastError());
 
-    double gm = 0.0;
     if (grad_W0) ADJCHK(hipMemcpy(grad_W0, a->LAM, 6 * N * sizeof(double), hipMemcpyDeviceToHost));
-    ADJCHK(hipMemcpy(&gm, a->gmu, sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> slots(a->ngmu);
+    ADJCHK(hipMemcpy(slots.data(), a->gmu, a->ngmu * sizeof(double), hipMemcpyDeviceToHost));
+    double gm = 0.0;
+    for (double v : slots) gm += v;
     if (grad_mu) *grad_mu = gm;
     if (grad_angle1) {
         // mu = |tan a| L/H  ->  dmu/da = sign(tan a) (1 + tan^2 a) L/H

@@ -98,6 +98,8 @@ hipError_t adj_reverse_step_f32_fused(const Consts& k, const float* W, const dou
 hipError_t adj_reverse_step_f64(const Consts& k, const double* W, const double* H, double* P, double* ACC,
                                 double* AB, double* LAM, double* gmu, const double* dt_hist, int step, hipStream_t s);
 hipError_t adj_inject(double* LAM, size_t N, const int* gauges, int ng, const double* G, hipStream_t s);
+// d/dmu slots (one per workgroup of the largest reverse-step launch)
+size_t adj_gmu_slots(const Consts& k);
 hipError_t adj_set_step(Ctl* ctl, int step, hipStream_t s);
 
 struct AdjointHooks {
