@@ -111,6 +111,18 @@ int lhysea_get_gauges(lhysea_sim *s, double *out, int32_t *nrec);
 int lhysea_dt_history(lhysea_sim *s, double *out, int32_t max_steps, int32_t *nrec);
 /* LHYSEA_KERNEL_FUSED or LHYSEA_KERNEL_FOURPASS */
 int lhysea_set_kernel(lhysea_sim *s, int32_t mode);
+/* FP32C step by tile class (no counterpart in the reference; same results
+ * either way).  The grid is cut into 62 x 14 class tiles; a tile whose cells
+ * and 1-cell halo hold no sediment (h2 = 0, q2 = 0) takes a reduced kernel
+ * (single layer in LDS and registers, twice the occupancy), the others the
+ * general two-layer tile function.  mode -1 (default): the split launch is
+ * used when at least 3000 tiles are sediment-free (its general launch costs
+ * a fixed latency), chosen again every 1024 steps and every run_until
+ * chunk; 0: every tile general; 1: always split.  lhysea_tile_classes
+ * reports the class tiles, the general ones of the next step and whether
+ * the last launches were split. */
+int lhysea_set_tile_split(lhysea_sim *s, int32_t mode);
+int lhysea_tile_classes(lhysea_sim *s, int32_t *tiles, int32_t *general, int32_t *split);
 /* Time nsteps steps with HIP events on the handle's stream: total wall time
  * of the whole sequence and the average duration of the main step kernel
  * (bracketed launch by launch in a second pass when kernel_ms is not NULL). */

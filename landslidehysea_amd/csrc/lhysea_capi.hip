@@ -62,6 +62,12 @@ struct lhysea_sim {
     int hcap = 0;
     double* dt_seq = nullptr;    // frozen dt sequence (adjoint replay)
     lhysea_adj_work* adj = nullptr;  // adjoint forward pass awaiting its reverse sweep
+    // FP32C: tile classes of the step and the launch split they select
+    int* tcls = nullptr;
+    int* tlist = nullptr;
+    int ntiles = 0;
+    int split = 0;
+    int split_mode = -1;         // -1 chosen per launch piece, 0 never split, 1 always
 };
 
 
@@ -119,7 +125,31 @@ static StepArgs step_args(lhysea_sim* s)
     a.dt_hist = s->dt_hist;
     a.hcap = s->hcap;
     a.t_end = INFINITY;
+    a.tcls = s->tcls;
+    a.tlist = s->tlist;
+    a.split = s->split;
     return a;
+}
+
+// Launch split of the next steps from the control block just read: the
+// sediment-free launch pays when enough tiles are sediment-free.
+static void choose_split(lhysea_sim* s, const Ctl& c)
+{
+    s->split = (s->split_mode >= 0) ? s->split_mode
+                                    : (s->ntiles - (int)c.gcount[c.step & 1]) >= f32c_split_min_bare();
+}
+
+// Tile classes of the current FP32C state at the current step; needed after
+// every change of the state or step count other than a step.
+static hipError_t classify(lhysea_sim* s)
+{
+    if (s->p.precision != LHYSEA_FP32C) return hipSuccess;
+    hipError_t e = f32c_classify(s->k, s->b32.W[s->cur], step_args(s), s->stream);
+    Ctl c;
+    if (e == hipSuccess) e = hipMemcpyAsync(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) choose_split(s, c);
+    return e;
 }
 
 static StepArgs launch_args(lhysea_sim* s, hipEvent_t ev0, hipEvent_t ev1)
@@ -221,6 +251,12 @@ int lhysea_create(const lhysea_params* p, lhysea_sim** out)
                 return cleanup(LHYSEA_ERR_GPU_MEMORY, "device state allocation failed");
         if (hipMalloc(&s->b32.H, N * sizeof(double)) != hipSuccess)
             return cleanup(LHYSEA_ERR_GPU_MEMORY, "device allocation failed");
+        // tile classes of the step: two parities of stamps and lists
+        s->ntiles = f32c_tiles(s->k);
+        const size_t nt = (size_t)s->ntiles;
+        if (hipMalloc(&s->tcls, 2 * nt * sizeof(int)) != hipSuccess ||
+            hipMalloc(&s->tlist, 2 * nt * sizeof(int)) != hipSuccess)
+            return cleanup(LHYSEA_ERR_GPU_MEMORY, "device allocation failed");
     }
     if (hipMalloc(&s->ctl, sizeof(Ctl)) != hipSuccess || hipMalloc(&s->emax, 2 * N * sizeof(double)) != hipSuccess)
         return cleanup(LHYSEA_ERR_GPU_MEMORY, "device allocation failed");
@@ -252,6 +288,8 @@ int lhysea_destroy(lhysea_sim* s)
     hipFree(s->gseries);
     hipFree(s->dt_hist);
     hipFree(s->dt_seq);
+    hipFree(s->tcls);
+    hipFree(s->tlist);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
     return LHYSEA_OK;
@@ -289,6 +327,7 @@ int lhysea_upload(lhysea_sim* s, const double* W, const double* H)
     c.dt_cur = -1.0;
     HIPCHK(hipMemcpy(s->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice));
     s->gstart = 0;
+    HIPCHK(classify(s));
     return LHYSEA_OK;
 }
 
@@ -343,7 +382,19 @@ int lhysea_step(lhysea_sim* s, int nsteps)
         int rc = lhysea_initial_dt(s, nullptr);
         if (rc) return rc;
     }
-    return launch_steps(s, nsteps);
+    // pieces of at most 1024 steps, the launch split chosen again before each
+    for (int done = 0; done < nsteps;) {
+        if (done > 0) {
+            HIPCHK(hipMemcpyAsync(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
+            HIPCHK(hipStreamSynchronize(s->stream));
+        }
+        choose_split(s, c);
+        const int n = (nsteps - done < 1024) ? nsteps - done : 1024;
+        const int rc = launch_steps(s, n);
+        if (rc) return rc;
+        done += n;
+    }
+    return LHYSEA_OK;
 }
 
 int lhysea_run_until(lhysea_sim* s, double t_end, int32_t max_steps, int32_t* steps_done)
@@ -369,6 +420,7 @@ int lhysea_run_until(lhysea_sim* s, double t_end, int32_t max_steps, int32_t* st
             chunk = (est > 511.0) ? 512 : (int)est + 1;
         }
         if (chunk > max_steps - (c.step - step0)) chunk = max_steps - (c.step - step0);
+        choose_split(s, c);
         const int cur0 = s->cur, stepc = c.step;
         int rc = launch_steps(s, chunk, nullptr, nullptr, device_stop ? t_end : INFINITY);
         if (rc) return rc;
@@ -485,6 +537,26 @@ int lhysea_set_kernel(lhysea_sim* s, int32_t mode)
     return LHYSEA_OK;
 }
 
+int lhysea_set_tile_split(lhysea_sim* s, int32_t mode)
+{
+    if (!s || mode < -1 || mode > 1) return fail(LHYSEA_ERR_ARGUMENT, "bad handle or split mode");
+    if (s->adj) return adj_busy();
+    s->split_mode = mode;
+    return LHYSEA_OK;
+}
+
+int lhysea_tile_classes(lhysea_sim* s, int32_t* tiles, int32_t* general, int32_t* split)
+{
+    if (!s) return fail(LHYSEA_ERR_ARGUMENT, "null handle");
+    Ctl c;
+    HIPCHK(hipMemcpyAsync(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (tiles) *tiles = s->ntiles;
+    if (general) *general = (s->p.precision == LHYSEA_FP32C) ? (int32_t)c.gcount[c.step & 1] : 0;
+    if (split) *split = s->split;
+    return LHYSEA_OK;
+}
+
 int lhysea_time_steps(lhysea_sim* s, int32_t nsteps, double* total_ms, double* kernel_ms)
 {
     if (!s || nsteps < 1) return fail(LHYSEA_ERR_ARGUMENT, "bad arguments");
@@ -496,6 +568,7 @@ int lhysea_time_steps(lhysea_sim* s, int32_t nsteps, double* total_ms, double* k
         int rc = lhysea_initial_dt(s, nullptr);
         if (rc) return rc;
     }
+    choose_split(s, c);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -598,6 +671,7 @@ hipError_t hook_advance(void* c, int k0, int n)
 {
     lhysea_adj_work* a = (lhysea_adj_work*)c;
     hipError_t e = adj_set_step(a->s->ctl, k0, a->s->stream);
+    if (e == hipSuccess) e = classify(a->s);   // the restored state's tile classes
     if (e != hipSuccess) return e;
     return launch_steps(a->s, n) == LHYSEA_OK ? hipSuccess : hipErrorUnknown;
 }

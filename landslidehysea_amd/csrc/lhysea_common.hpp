@@ -37,6 +37,7 @@ struct Ctl {
     double dt_cur;                  // dt of the running step
     double t;                       // simulated time at the start of the running step
     double maxA_last;               // max A of the last completed step (diagnostic)
+    unsigned int gcount[2];         // general tiles listed for the steps of parity 0 / 1 (FP32C)
 };
 
 // dt from the wave-speed max: min_i dt_i with dt_i = 2 CFL area / A_i, or 1e30
@@ -46,5 +47,13 @@ __host__ __device__ inline double dt_from_maxA(double maxA, double cfl, double a
 {
     return (maxA < LH_EPS) ? LH_BIG : (2.0 * cfl * area) / maxA;
 }
+
+// Gauge output of one step: eta = h1 + h2 - H of the new state at the gauge
+// cells, written by the workgroup that updates the cell (out null: none).
+struct GaugeOut {
+    const int* cells;
+    int ng;
+    double* out;
+};
 
 }  // namespace lhysea

@@ -62,6 +62,26 @@ __device__ __forceinline__ K32 make_k32(const Consts& k, double dt)
     return c;
 }
 
+__device__ __forceinline__ KCell make_kcell(const Consts& k, const K32& kc, double dt)
+{
+    KCell kk;
+    kk.eps = kc.eps;
+    kk.eps4 = kc.eps4;
+    kk.r = kc.r;
+    kk.g = kc.g;
+    kk.mu = kc.mu;
+    kk.mfc = (float)k.mfc;
+    kk.mf0 = (float)k.mf0;
+    kk.mfs = (float)k.mfs;
+    kk.vmax1 = (float)k.vmax1;
+    kk.vmax2 = (float)k.vmax2;
+    kk.dtf = kc.dtf;
+    kk.valf = (float)(dt / k.area);
+    kk.ieps = (float)(1.0 / k.eps_h);
+    kk.ten_eps = kc.ten_eps;
+    return kk;
+}
+
 // ---------------------------------------------------------------- initial dt
 __global__ void __launch_bounds__(256) k_dt0(Consts k, const float* __restrict__ W, Ctl* ctl)
 {
@@ -115,32 +135,6 @@ __global__ void k_dt0_finalize(Consts k, Ctl* ctl)
     }
 }
 
-__global__ void k_finalize(Consts k, const float* __restrict__ Wn, const double* __restrict__ H, StepArgs a)
-{
-    Ctl* ctl = a.ctl;
-    if (!(ctl->t < a.t_end)) return;
-    const size_t N = (size_t)k.nx * k.ny;
-    const int s = ctl->step;
-    const double dt = step_dt(ctl, a.dt_seq);
-    if (a.gseries && s < a.gcap)
-        for (int g = threadIdx.x; g < a.ng; g += blockDim.x) {
-            const size_t c = (size_t)a.gauges[g];
-            const double h1 = (double)Wn[c] + (double)Wn[6 * N + c];
-            const double h2 = (double)Wn[3 * N + c] + (double)Wn[9 * N + c];
-            a.gseries[(size_t)s * a.ng + g] = h1 + h2 - H[c];
-        }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (a.dt_hist && s < a.hcap) a.dt_hist[s] = dt;
-        double maxA = __longlong_as_double((long long)ctl->maxA_next);
-        ctl->t = ctl->t + dt;
-        ctl->dt_cur = dt_from_maxA(maxA, k.cfl, k.area);
-        ctl->maxA_last = maxA;
-        ctl->maxA_next = 0ull;
-        ctl->step = s + 1;
-    }
-}
-
 // Both cells of an edge without sediment: h2 exactly 0 (FP64) and q2 = 0.
 // When every edge of a wavefront is such an edge it takes edge1(), which
 // produces no layer-2 fluxes; the layer-2 accumulators then stay untouched.
@@ -160,6 +154,44 @@ __device__ __forceinline__ void accumulate(float (&sA)[7][TY][CW], int r, int c,
     sA[6][r][c] += lam;
 }
 
+// Class tiles whose tile + halo box holds a cell at (lx, ly) of a FTX x FTY
+// class tile, as bits (dy + 1) * 3 + (dx + 1) for the neighbour at (dx, dy):
+// the tile itself, and the neighbours across the edges and corners the cell
+// lies on.
+constexpr int FTX = 62, FTY = 14, FNT = 512;
+// general sub-tiles: width, threads, and the fixed grid walking them (four
+// workgroups per CU of an MI355X)
+constexpr int GSTX = 16, GNT = 256, GGRID = 1024;
+__device__ __forceinline__ int halo_mask(int lx, int ly)
+{
+    const int x0 = (lx == 0) ? 0 : 1, x1 = (lx == FTX - 1) ? 2 : 1;
+    const int y0 = (ly == 0) ? 0 : 1, y1 = (ly == FTY - 1) ? 2 : 1;
+    int m = 0;
+    for (int y = y0; y <= y1; y++)
+        for (int x = x0; x <= x1; x++) m |= 1 << (3 * y + x);
+    return m;
+}
+
+// Gauges inside the tile (columns below xend): eta = h1 + h2 - H from the
+// new h1 (sh1), h2 (sh2, or zero when null) and the depth the block staged,
+// after a barrier that publishes the threads' new values.
+// LDS planes are row-major with row length cwp, the tile's cell (r, c) at
+// row r + 1, column c + 1.
+template <int TX, int TY, int NT>
+__device__ __forceinline__ void gauges_of_tile(const GaugeOut& go, const double* sh1, const double* sh2,
+                                               const double* sH, int cwp, int nx, int i0, int j0, int xend)
+{
+    __syncthreads();
+    for (int gi = threadIdx.x; gi < go.ng; gi += NT) {
+        const int cell = go.cells[gi];
+        const int r = cell / nx - j0, c = cell % nx - i0;
+        if (r >= 0 && r < TY && c >= 0 && c < TX && i0 + c < xend) {
+            const int at = (r + 1) * cwp + c + 1;
+            go.out[gi] = (sh1[at] + (sh2 ? sh2[at] : 0.0)) - sH[at];
+        }
+    }
+}
+
 // ----------------------------------------------------------------- fused step
 // One even-aligned TX x TY tile of one step (origin i0, j0): stage the tile
 // and its 1-cell halo, run the four edge classes in the reference's order,
@@ -169,6 +201,10 @@ __device__ __forceinline__ void accumulate(float (&sA)[7][TY][CW], int r, int c,
 // sA (FP32 accumulators: dh1,dq1x,dq1y,dh2,dq2x,dq2y,A).
 // Every cell receives its class-C1..C4 contributions in that order whatever
 // the tile shape, so the result does not depend on TX, TY or NT.
+// Only cells with column < xend are written.  go: gauges of the new state.
+// smask (optional): OR of halo_mask() over this thread's cells that keep
+// sediment (h2 or q2 nonzero) in the new state, with the cell's position in
+// the FTX x FTY class tile at (cx0, cy0).
 // RECOMP: instead of updating the state, write the partial h1/h2 sums that
 // classes 2..4 read (P, [3][2][N]) and the final flux sums (ACC, [6][N]) of
 // the tile's cells: the forward recomputation of the discrete adjoint.
@@ -176,7 +212,9 @@ template <int TX, int TY, int NT, bool RECOMP = false>
 __device__ __forceinline__ double tile_step(const Consts& k, const float* __restrict__ W, float* __restrict__ Wn,
                                             const double* __restrict__ H, double dt, double tnow,
                                             double* __restrict__ emax, int i0, int j0,
-                                            float* __restrict__ P = nullptr, float* __restrict__ ACC = nullptr)
+                                            float* __restrict__ P = nullptr, float* __restrict__ ACC = nullptr,
+                                            int xend = 0x7fffffff, GaugeOut go = GaugeOut{nullptr, 0, nullptr},
+                                            int* smask = nullptr, int cx0 = 0, int cy0 = 0)
 {
     static_assert(TX % 2 == 0 && TY % 2 == 0, "tile origins must stay even");
     // LDS rows are padded to an odd length (CW + 1 words) so that the
@@ -413,21 +451,7 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
     }
 
     // compensated cell update of the tile's own cells
-    KCell kk;
-    kk.eps = kc.eps;
-    kk.eps4 = kc.eps4;
-    kk.r = kc.r;
-    kk.g = kc.g;
-    kk.mu = kc.mu;
-    kk.mfc = (float)k.mfc;
-    kk.mf0 = (float)k.mf0;
-    kk.mfs = (float)k.mfs;
-    kk.vmax1 = (float)k.vmax1;
-    kk.vmax2 = (float)k.vmax2;
-    kk.dtf = kc.dtf;
-    kk.valf = (float)(dt / k.area);
-    kk.ieps = (float)(1.0 / k.eps_h);
-    kk.ten_eps = kc.ten_eps;
+    const KCell kk = make_kcell(k, kc, dt);
     double Amax = 0.0;
 #pragma unroll
     for (int it = 0; it < CU_IT; it++) {
@@ -435,7 +459,7 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
         if (e >= TX * TY) continue;
         const int r = e / TX, c = e % TX;
         const int gj = j0 + r, gi = i0 + c;
-        if (gj >= ny || gi >= nx) continue;
+        if (gj >= ny || gi >= nx || gi >= xend) continue;
         const size_t g = (size_t)gj * nx + gi;
         Pair o[6];
         const double h1d = sh1[r + 1][c + 1], h2d = sh2[r + 1][c + 1];
@@ -466,6 +490,9 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
                 Wn[(3 + f) * N + g] = 0.0f;
                 Wn[(9 + f) * N + g] = 0.0f;
             }
+            // the new h1 (h2 stays 0) replaces this cell's staged one, which
+            // no other thread reads any more
+            if (go.out) sh1[r + 1][c + 1] = (double)o[0].hi + (double)o[0].lo;
         } else {
             o[3].hi = (float)h2d; o[3].lo = (float)(h2d - (double)o[3].hi);
             o[4] = Pair{sq[2][r + 1][c + 1], W[10 * N + g]};
@@ -476,33 +503,474 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
                 Wn[f * N + g] = o[f].hi;
                 Wn[(6 + f) * N + g] = o[f].lo;
             }
+            if (go.out) {
+                sh1[r + 1][c + 1] = (double)o[0].hi + (double)o[0].lo;
+                sh2[r + 1][c + 1] = (double)o[3].hi + (double)o[3].lo;
+            }
+            if (smask && (o[3].hi != 0.0f || o[4].hi != 0.0f || o[5].hi != 0.0f))
+                *smask |= halo_mask(gi - cx0, gj - cy0);
         }
     }
+    if (go.out) gauges_of_tile<TX, TY, NT>(go, &sh1[0][0], &sh2[0][0], &sH[0][0], CWP, nx, i0, j0, xend);
     return Amax;
 }
 
-// Single-simulation step: one block per tile, tiles of one XCD contiguous
-// (blocks are dispatched round-robin over the 8 XCDs, so XCD x runs blocks
-// x, x+8, ...; giving it one run of tiles keeps shared halo rows in its L2).
+// ------------------------------------------------------- sediment-free tile
+// tile_step() for a tile whose cells and halo carry no sediment: h2 = 0
+// (value and compensation) and q2 = 0 in every staged cell.  There every
+// edge takes edge1() and every cell cell_update1(), exactly the arithmetic
+// tile_step() performs on such a tile, so the results are the same; the
+// layer-2 planes are neither read nor kept in LDS (zeros are written), which
+// halves the LDS footprint and the registers the general two-layer path needs.
+// A tile whose class entry *cls equals s belongs to the general kernel: the
+// block returns false before its first LDS store (its loads were issued).
 template <int TX, int TY, int NT>
-__global__ void __launch_bounds__(NT) k_fused(Consts k, const float* __restrict__ W, float* __restrict__ Wn,
-                                              const double* __restrict__ H, StepArgs a, int tiles_x, int tiles_y)
+__device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __restrict__ W, float* __restrict__ Wn,
+                                               const double* __restrict__ H, double dt, double tnow,
+                                               double* __restrict__ emax, int i0, int j0, const int* cls, int s,
+                                               GaugeOut go, double& Amax)
+{
+    static_assert(TX % 2 == 0 && TY % 2 == 0, "tile origins must stay even");
+    constexpr int CW = TX + 2, RW = TY + 2, CWP = CW + 1;
+    __shared__ double sH[RW][CWP];
+    __shared__ double sh1[RW][CWP];
+    __shared__ float sq[2][RW][CWP];
+    __shared__ float sA[4][TY][CWP];   // dh1, dq1x, dq1y, wave-speed sum
+
+    const int nx = k.nx, ny = k.ny;
+    const size_t N = (size_t)nx * ny;
+    const K32 kc = make_k32(k, dt);
+    const int tid = threadIdx.x;
+
+    constexpr int CU_IT = (TX * TY + NT - 1) / NT, ST_IT = (RW * CW + NT - 1) / NT;
+    float lo1x[CU_IT], lo1y[CU_IT];
+#pragma unroll
+    for (int it = 0; it < CU_IT; it++) {
+        const int e = tid + it * NT;
+        const int r = e / TX, c = e % TX;
+        lo1x[it] = lo1y[it] = 0.0f;
+        if (e < TX * TY && j0 + r < ny && i0 + c < nx) {
+            const size_t g = (size_t)(j0 + r) * nx + (i0 + c);
+            lo1x[it] = W[7 * N + g];
+            lo1y[it] = W[8 * N + g];
+        }
+    }
+    {
+        double vH[ST_IT], vh1[ST_IT];
+        float vq[ST_IT][2];
+        bool in[ST_IT];
+#pragma unroll
+        for (int it = 0; it < ST_IT; it++) {
+            const int e = tid + it * NT;
+            const int r = e / CW, c = e % CW;
+            const int gj = j0 + r - 1, gi = i0 + c - 1;
+            in[it] = e < RW * CW && gj >= 0 && gj < ny && gi >= 0 && gi < nx;
+            if (in[it]) {
+                const size_t g = (size_t)gj * nx + gi;
+                vH[it] = H[g];
+                vh1[it] = (double)W[g] + (double)W[6 * N + g];
+                vq[it][0] = W[N + g];
+                vq[it][1] = W[2 * N + g];
+            }
+        }
+        if (*cls == s) return false;
+#pragma unroll
+        for (int it = 0; it < ST_IT; it++) {
+            if (!in[it]) continue;
+            const int e = tid + it * NT;
+            const int r = e / CW, c = e % CW;
+            sH[r][c] = vH[it];
+            sh1[r][c] = vh1[it];
+            sq[0][r][c] = vq[it][0];
+            sq[1][r][c] = vq[it][1];
+        }
+    }
+    for (int e = tid; e < TY * CW; e += NT) {
+        const int r = e / CW, c = e % CW;
+#pragma unroll
+        for (int f = 0; f < 4; f++) sA[f][r][c] = 0.0f;
+    }
+    __syncthreads();
+
+    auto loadW = [&](int r, int c, float* w) {
+        w[0] = (float)sh1[r][c];
+        w[1] = sq[0][r][c];
+        w[2] = sq[1][r][c];
+    };
+    auto acc = [&](int r, int c, const float* F, float lam) {
+#pragma unroll
+        for (int f = 0; f < 3; f++) sA[f][r][c] -= F[f];
+        sA[3][r][c] += lam;
+    };
+
+    // y-edge classes C1 then C2
+#pragma unroll 1
+    for (int par = 0; par < 2; par++) {
+        const int nrow = (par == 0) ? (TY / 2 + 1) : (TY / 2);
+        for (int e = tid; e < nrow * CW; e += NT) {
+            const int c = e % CW, jl = par + 2 * (e / CW);
+            const int gi = i0 + c - 1, je = j0 + jl;
+            if (gi < 0 || gi >= nx || je > ny) continue;
+            float W0[3], W1[3];
+            double h10, h11, H0, H1;
+            float a1h1 = 0.0f, n1y;
+            int r0;
+            bool interior;
+            if (je == 0 || je == ny) {
+                r0 = (je == 0) ? jl : jl - 1;
+                const float bd = (float)((je == 0) ? k.borde_sup : k.borde_inf);
+                loadW(r0 + 1, c, W0);
+                W1[0] = W0[0]; W1[1] = W0[1]; W1[2] = W0[2] * bd;
+                h10 = h11 = sh1[r0 + 1][c];
+                H0 = H1 = sH[r0 + 1][c];
+                n1y = (je == 0) ? -1.0f : 1.0f;
+                interior = false;
+            } else {
+                r0 = jl - 1;
+                loadW(r0 + 1, c, W0);
+                loadW(r0 + 2, c, W1);
+                h10 = sh1[r0 + 1][c];
+                h11 = sh1[r0 + 2][c];
+                H0 = sH[r0 + 1][c];
+                H1 = sH[r0 + 2][c];
+                n1y = 1.0f;
+                interior = true;
+            }
+            const bool has0 = (r0 >= 0 && r0 < TY), has1 = interior && (r0 + 1 < TY);
+            const float a0h1 = has0 ? sA[0][r0][c] : 0.0f;
+            if (has1) a1h1 = sA[0][r0 + 1][c];
+            EdgeFlux F;
+            if (edge1(kc, W0, W1, h10, h11, H0, H1, 0.0f, n1y, kc.len_y, a0h1, a1h1, interior, F)) {
+                if (has0) acc(r0, c, F.Fm, F.lam);
+                if (has1) acc(r0 + 1, c, F.Fp, F.lam);
+            }
+        }
+        __syncthreads();
+    }
+
+    // x-edge classes C3 then C4, 16-lane segments of 2 rows x 8 edge columns
+    constexpr int NB = (TX / 2 + 1 + 7) / 8;
+#pragma unroll 1
+    for (int par = 0; par < 2; par++) {
+        const int ncol = (par == 0) ? (TX / 2 + 1) : (TX / 2);
+        for (int e = tid; e < (TY / 2) * NB * 16; e += NT) {
+            const int seg = e >> 4, w = e & 15;
+            const int r = 2 * (seg / NB) + (w >> 3), j = 8 * (seg % NB) + (w & 7);
+            if (j >= ncol) continue;
+            const int il = par + 2 * j;
+            const int gj = j0 + r, ie = i0 + il;
+            if (gj >= ny || ie > nx) continue;
+            float W0[3], W1[3];
+            double h10, h11, H0, H1;
+            float a1h1 = 0.0f, n1x;
+            int c0;
+            bool interior;
+            if (ie == 0 || ie == nx) {
+                c0 = (ie == 0) ? il + 1 : il;
+                const float bd = (float)((ie == 0) ? k.borde_izq : k.borde_der);
+                loadW(r + 1, c0, W0);
+                W1[0] = W0[0]; W1[1] = W0[1] * bd; W1[2] = W0[2];
+                h10 = h11 = sh1[r + 1][c0];
+                H0 = H1 = sH[r + 1][c0];
+                n1x = (ie == 0) ? -1.0f : 1.0f;
+                interior = false;
+            } else {
+                c0 = il;
+                loadW(r + 1, c0, W0);
+                loadW(r + 1, c0 + 1, W1);
+                h10 = sh1[r + 1][c0];
+                h11 = sh1[r + 1][c0 + 1];
+                H0 = sH[r + 1][c0];
+                H1 = sH[r + 1][c0 + 1];
+                n1x = 1.0f;
+                interior = true;
+            }
+            const float a0h1 = sA[0][r][c0];
+            if (interior) a1h1 = sA[0][r][c0 + 1];
+            EdgeFlux F;
+            if (edge1(kc, W0, W1, h10, h11, H0, H1, n1x, 0.0f, kc.len_x, a0h1, a1h1, interior, F)) {
+                acc(r, c0, F.Fm, F.lam);
+                if (interior) acc(r, c0 + 1, F.Fp, F.lam);
+            }
+        }
+        __syncthreads();
+    }
+
+    // cell update of the tile's own cells (layer 2 stays zero)
+    const KCell kk = make_kcell(k, kc, dt);
+    Amax = 0.0;
+#pragma unroll
+    for (int it = 0; it < CU_IT; it++) {
+        const int e = tid + it * NT;
+        if (e >= TX * TY) continue;
+        const int r = e / TX, c = e % TX;
+        const int gj = j0 + r, gi = i0 + c;
+        if (gj >= ny || gi >= nx) continue;
+        const size_t g = (size_t)gj * nx + gi;
+        Pair o[3];
+        const double h1d = sh1[r + 1][c + 1];
+        o[0].hi = (float)h1d; o[0].lo = (float)(h1d - (double)o[0].hi);
+        o[1] = Pair{sq[0][r + 1][c + 1], lo1x[it]};
+        o[2] = Pair{sq[1][r + 1][c + 1], lo1y[it]};
+        const float ac[3] = {sA[0][r][c + 1], sA[1][r][c + 1], sA[2][r][c + 1]};
+        Amax = fmax(Amax, (double)sA[3][r][c + 1]);
+        if (emax) {
+            const double v = h1d - sH[r + 1][c + 1];
+            if (v > emax[2 * g]) {
+                emax[2 * g] = v;
+                emax[2 * g + 1] = tnow;
+            }
+        }
+        cell_update1(kk, o, ac);
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            Wn[f * N + g] = o[f].hi;
+            Wn[(6 + f) * N + g] = o[f].lo;
+            Wn[(3 + f) * N + g] = 0.0f;
+            Wn[(9 + f) * N + g] = 0.0f;
+        }
+        // the new h1 replaces this cell's staged one (read by no other thread now)
+        if (go.out) sh1[r + 1][c + 1] = (double)o[0].hi + (double)o[0].lo;
+    }
+    if (go.out) gauges_of_tile<TX, TY, NT>(go, &sh1[0][0], nullptr, &sH[0][0], CWP, nx, i0, j0, 0x7fffffff);
+    return true;
+}
+
+// ------------------------------------------------------------ tile classes
+// The single-simulation step is split by tile class.  The class tiles are the
+// FTX x FTY tiles of the grid.  A class tile is general at step s when its
+// tile + halo box holds a cell with sediment in the state of step s: its own
+// cells, or the edge and corner cells of its neighbours that form its halo.
+// cls[(s & 1) * nt + t] == s marks class tile t general at step s and
+// list[(s & 1) * nt + i], i < ctl->gcount[s & 1], enumerates those tiles; the
+// sediment-free kernel skips them and the general kernel takes them.  A cell
+// that keeps sediment through a general update marks, for step s + 1 (the
+// other parity, so the running step's entries stay intact), its own tile
+// and the neighbours whose halo it lies in.  A sediment-free tile cannot gain
+// sediment in one step (no edge of its box carries any), so it marks nothing.
+struct TileCls {
+    int* cls;
+    int* list;
+    int tiles_x, tiles_y;
+};
+
+// Marks, for step s, the class tiles of the 3 x 3 neighbourhood of (tx, ty)
+// whose bit is set in mask (bit (dy + 1) * 3 + (dx + 1)).  Called by the
+// first 9 threads of a wavefront with the same mask, thread b taking bit b,
+// so the atomics of the nine tiles are in flight together.
+__device__ __forceinline__ void mark_general(const TileCls& tc, Ctl* ctl, int tx, int ty, int s, int mask)
+{
+    const int bit = threadIdx.x & 63;
+    if (bit >= 9 || !(mask & (1 << bit))) return;
+    const int nt = tc.tiles_x * tc.tiles_y, base = (s & 1) * nt;
+    const int x = tx + bit % 3 - 1, y = ty + bit / 3 - 1;
+    if (x < 0 || y < 0 || x >= tc.tiles_x || y >= tc.tiles_y) return;
+    const int t = y * tc.tiles_x + x;
+    // the first marker of t for step s appends it
+    if (atomicMax(&tc.cls[base + t], s) < s) tc.list[base + (int)atomicAdd(&ctl->gcount[s & 1], 1u)] = t;
+}
+
+// Workgroup end of a general tile: thread 0 issues the atomicMax of the
+// wave-speed max v and marks the next step's general tiles from the OR of
+// the halo masks m (one barrier for both reductions).
+template <int NT>
+__device__ __forceinline__ void general_tile_end(double v, int m, const TileCls& tc, Ctl* ctl, int tx, int ty,
+                                                 int s_next)
+{
+    __shared__ double red[NT / 64];
+    __shared__ int ored[NT / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        v = fmax(v, __shfl_xor(v, off, 64));
+        m |= __shfl_xor(m, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[wid] = v;
+        ored[wid] = m;
+    }
+    __syncthreads();
+    if (wid == 0) {
+        // every lane of wavefront 0 forms both reductions
+        for (int i = 1; i < NT / 64; i++) {
+            v = fmax(v, red[i]);
+            m |= ored[i];
+        }
+        if (lane == 0 && v > 0.0) atomicMax(&ctl->maxA_next, (unsigned long long)__double_as_longlong(v));
+        if (m) mark_general(tc, ctl, tx, ty, s_next, m);
+    }
+}
+
+// Block-wide OR of m; every thread calls it.
+__device__ __forceinline__ int block_or(int m)
+{
+    __shared__ int acc;
+    if (threadIdx.x == 0) acc = 0;
+    __syncthreads();
+    if (m) atomicOr(&acc, m);
+    __syncthreads();
+    const int r = acc;
+    __syncthreads();
+    return r;
+}
+
+// Sediment-free tiles of a step: one block per class tile, tiles of one XCD
+// contiguous (blocks are dispatched round-robin over the 8 XCDs, so XCD x
+// runs blocks x, x+8, ...; giving it one run of tiles keeps shared halo rows
+// in its L2).
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const float* __restrict__ W, float* __restrict__ Wn,
+                                                   const double* __restrict__ H, StepArgs a, TileCls tc)
 {
     if (!(a.ctl->t < a.t_end)) return;     // the run has reached its end time
-    const int nb = tiles_x * tiles_y;
+    const int nb = tc.tiles_x * tc.tiles_y;
     int b;
     {
         const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
         const int q = nb >> 3, rem = nb & 7;
         b = xcd * q + (xcd < rem ? xcd : rem) + slot;
     }
-    const int tx = b % tiles_x, ty = b / tiles_x;
+    const int tx = b % tc.tiles_x, ty = b / tc.tiles_x;
+    const int s = a.ctl->step;
     const double dt = step_dt(a.ctl, a.dt_seq);
-    const double Amax = tile_step<TX, TY, NT>(k, W, Wn, H, dt, a.ctl->t, a.emax, tx * TX, ty * TY);
+    const GaugeOut go{a.gauges, a.ng, (a.gseries && s < a.gcap) ? a.gseries + (size_t)s * a.ng : nullptr};
+    double Amax;
+    if (!tile_step_bare<TX, TY, NT>(k, W, Wn, H, dt, a.ctl->t, a.emax, tx * TX, ty * TY, tc.cls + (s & 1) * nb + b,
+                                    s, go, Amax))
+        return;
     block_max_atomic<NT>(Amax, &a.ctl->maxA_next);
 }
 
-constexpr int FTX = 62, FTY = 14, FNT = 512;
+// General tiles of a step, then the step's close.  Each general class tile
+// is split into STX-wide sub-tiles (one work item each, the last one clipped
+// at the class tile's edge) so that the few general tiles spread over many
+// workgroups; a fixed grid walks the work items.  Cells that keep sediment
+// mark the next step's general tiles.  Every workgroup then adds its
+// wave-speed max and counts itself done; the last one closes the step
+// (src/GPU/ShallowWater.cu:431-442): dt history, t += dt, the lagged dt of
+// the next step from the max of both launches, the consumed list and the
+// counters cleared.  Workgroups beyond the work items leave at once.  Its count increment uses the value its atomicMax
+// returned, so it is issued once that max has been performed, and the
+// closing workgroup reads the max with an atomic.  The sediment-free launch
+// ran before this one, so its maxima are in place.
+template <int STX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_fused_gen(Consts k, const float* __restrict__ W, float* __restrict__ Wn,
+                                                  const double* __restrict__ H, StepArgs a, TileCls tc)
+{
+    constexpr int NSUB = (FTX + STX - 1) / STX;
+    static_assert(TY == FTY && STX % 2 == 0, "sub-tiles of a class tile");
+    Ctl* ctl = a.ctl;
+    if (!(ctl->t < a.t_end)) return;
+    const int s = ctl->step;
+    const int n = NSUB * (int)ctl->gcount[s & 1];
+    // workgroups with work items (at least one, which then closes the step)
+    const unsigned part = (unsigned)max(min(n, (int)gridDim.x), 1);
+    if (blockIdx.x >= part) return;
+    const int nt = tc.tiles_x * tc.tiles_y;
+    const double dt = step_dt(ctl, a.dt_seq);
+    const double tnow = ctl->t;
+    const GaugeOut go{a.gauges, a.ng, (a.gseries && s < a.gcap) ? a.gseries + (size_t)s * a.ng : nullptr};
+    double Amax = 0.0;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int t = tc.list[(s & 1) * nt + i / NSUB], sub = i % NSUB;
+        const int tx = t % tc.tiles_x, ty = t / tc.tiles_x;
+        const int cx0 = tx * FTX, cy0 = ty * FTY;
+        int m = 0;
+        Amax = fmax(Amax, tile_step<STX, TY, NT>(k, W, Wn, H, dt, tnow, a.emax, cx0 + sub * STX, cy0, nullptr,
+                                                 nullptr, cx0 + FTX, go, &m, cx0, cy0));
+        // also orders this sub-tile's LDS reads before the next one's staging
+        m = block_or(m);
+        if (m && threadIdx.x < 64) mark_general(tc, ctl, tx, ty, s + 1, m);
+    }
+    // close of the step
+    __shared__ double red[NT / 64];
+    double v = Amax;
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NT / 64; w++) v = fmax(v, red[w]);
+        v = fmax(v, red[0]);
+        const unsigned long long old = atomicMax(&ctl->maxA_next, (unsigned long long)__double_as_longlong(v));
+        // bit 63 of a non-negative double is clear: the increment is 1
+        const unsigned done = atomicAdd(&ctl->blocks_done, 1u + (unsigned)(old >> 63));
+        if (done == part - 1) {
+            const double maxA = __longlong_as_double((long long)atomicAdd(&ctl->maxA_next, 0ull));
+            if (a.dt_hist && s < a.hcap) a.dt_hist[s] = dt;
+            ctl->t = tnow + dt;
+            ctl->dt_cur = dt_from_maxA(maxA, k.cfl, k.area);
+            ctl->maxA_last = maxA;
+            ctl->gcount[s & 1] = 0u;
+            atomicExch(&ctl->maxA_next, 0ull);
+            atomicExch(&ctl->blocks_done, 0u);
+            ctl->step = s + 1;
+        }
+    }
+}
+
+// Every class tile through the general tile function, one block per tile
+// (XCD-contiguous order), for grids where the sediment-free tiles are too
+// few to pay for the split; the classes are kept up to date all the same.
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_fused_all(Consts k, const float* __restrict__ W, float* __restrict__ Wn,
+                                                  const double* __restrict__ H, StepArgs a, TileCls tc)
+{
+    static_assert(TX == FTX && TY == FTY, "class tiles");
+    if (!(a.ctl->t < a.t_end)) return;
+    const int nb = tc.tiles_x * tc.tiles_y;
+    int b;
+    {
+        const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+        const int q = nb >> 3, rem = nb & 7;
+        b = xcd * q + (xcd < rem ? xcd : rem) + slot;
+    }
+    const int tx = b % tc.tiles_x, ty = b / tc.tiles_x;
+    const int s = a.ctl->step;
+    const double dt = step_dt(a.ctl, a.dt_seq);
+    const GaugeOut go{a.gauges, a.ng, (a.gseries && s < a.gcap) ? a.gseries + (size_t)s * a.ng : nullptr};
+    int m = 0;
+    const double Amax = tile_step<TX, TY, NT>(k, W, Wn, H, dt, a.ctl->t, a.emax, tx * TX, ty * TY, nullptr, nullptr,
+                                              0x7fffffff, go, &m, tx * TX, ty * TY);
+    general_tile_end<NT>(Amax, m, tc, a.ctl, tx, ty, s + 1);
+}
+
+// Close of a step run by k_fused_all (src/GPU/ShallowWater.cu:431-442).
+__global__ void k_close(Consts k, StepArgs a)
+{
+    Ctl* ctl = a.ctl;
+    if (threadIdx.x != 0 || !(ctl->t < a.t_end)) return;
+    const int s = ctl->step;
+    const double dt = step_dt(ctl, a.dt_seq);
+    if (a.dt_hist && s < a.hcap) a.dt_hist[s] = dt;
+    const double maxA = __longlong_as_double((long long)ctl->maxA_next);
+    ctl->t = ctl->t + dt;
+    ctl->dt_cur = dt_from_maxA(maxA, k.cfl, k.area);
+    ctl->maxA_last = maxA;
+    ctl->maxA_next = 0ull;
+    ctl->gcount[s & 1] = 0u;
+    ctl->step = s + 1;
+}
+
+// Class table of the state W at step ctl->step: one block per class tile;
+// cells with sediment mark as in a step.  The table and both counts are
+// cleared beforehand.
+template <int TX, int TY>
+__global__ void __launch_bounds__(256) k_classify(Consts k, const float* __restrict__ W, StepArgs a, TileCls tc)
+{
+    const int nx = k.nx, ny = k.ny;
+    const size_t N = (size_t)nx * ny;
+    const int t = blockIdx.x, tx = t % tc.tiles_x, ty = t / tc.tiles_x;
+    int m = 0;
+    for (int e = threadIdx.x; e < TX * TY; e += 256) {
+        const int ly = e / TX, lx = e % TX;
+        const int gj = ty * TY + ly, gi = tx * TX + lx;
+        if (gj >= ny || gi >= nx) continue;
+        const size_t g = (size_t)gj * nx + gi;
+        if (W[3 * N + g] != 0.0f || W[9 * N + g] != 0.0f || W[4 * N + g] != 0.0f || W[5 * N + g] != 0.0f)
+            m |= halo_mask(lx, ly);
+    }
+    m = block_or(m);
+    if (m && threadIdx.x < 64) mark_general(tc, a.ctl, tx, ty, a.ctl->step, m);
+}
 
 // Forward recomputation for the discrete adjoint: the step's partial and
 // final flux sums at state W with the recorded dt of step `step`.
@@ -589,18 +1057,51 @@ hipError_t f32c_initial_dt(const Consts& k, const float* W, Ctl* ctl, hipStream_
     return hipGetLastError();
 }
 
+int f32c_tiles(const Consts& k)
+{
+    return ((k.nx + f32::FTX - 1) / f32::FTX) * ((k.ny + f32::FTY - 1) / f32::FTY);
+}
+
+hipError_t f32c_classify(const Consts& k, const float* W, const StepArgs& a, hipStream_t s)
+{
+    const int nt = f32c_tiles(k);
+    const f32::TileCls tc{a.tcls, a.tlist, (k.nx + f32::FTX - 1) / f32::FTX, (k.ny + f32::FTY - 1) / f32::FTY};
+    hipError_t e = hipMemsetAsync(a.tcls, 0xFF, 2 * (size_t)nt * sizeof(int), s);   // no tile general (-1)
+    if (e == hipSuccess) e = hipMemsetAsync(a.ctl->gcount, 0, sizeof(a.ctl->gcount), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((f32::k_classify<f32::FTX, f32::FTY>), dim3((unsigned)nt), dim3(256), 0, s, k, W, a, tc);
+    return hipGetLastError();
+}
+
+// Split (a.split): the sediment-free launch, then the general sub-tiles and
+// the step's close, back to back on one stream.  The general launch takes
+// about one general sub-tile's latency (~25 us on an MI355X) however few
+// general tiles there are, so the split pays only when the sediment-free
+// tiles are many; otherwise every tile takes the general tile function.
 hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const double* H, const StepArgs& a,
                            hipStream_t s)
 {
     const int tiles_x = (k.nx + f32::FTX - 1) / f32::FTX;
     const int tiles_y = (k.ny + f32::FTY - 1) / f32::FTY;
+    const int nt = tiles_x * tiles_y;
+    const f32::TileCls tc{a.tcls, a.tlist, tiles_x, tiles_y};
     if (a.ev0) hipEventRecord(a.ev0, s);
-    hipLaunchKernelGGL((f32::k_fused<f32::FTX, f32::FTY, f32::FNT>), dim3((unsigned)(tiles_x * tiles_y)),
-                       dim3(f32::FNT), 0, s, k, W, Wn, H, a, tiles_x, tiles_y);
-    if (a.ev1) hipEventRecord(a.ev1, s);
-    hipLaunchKernelGGL(f32::k_finalize, dim3(1), dim3(64), 0, s, k, Wn, H, a);
+    if (a.split) {
+        hipLaunchKernelGGL((f32::k_fused_bare<f32::FTX, f32::FTY, f32::FNT>), dim3((unsigned)nt), dim3(f32::FNT), 0,
+                           s, k, W, Wn, H, a, tc);
+        hipLaunchKernelGGL((f32::k_fused_gen<f32::GSTX, f32::FTY, f32::GNT>), dim3(f32::GGRID), dim3(f32::GNT), 0, s,
+                           k, W, Wn, H, a, tc);
+        if (a.ev1) hipEventRecord(a.ev1, s);
+    } else {
+        hipLaunchKernelGGL((f32::k_fused_all<f32::FTX, f32::FTY, f32::FNT>), dim3((unsigned)nt), dim3(f32::FNT), 0, s,
+                           k, W, Wn, H, a, tc);
+        if (a.ev1) hipEventRecord(a.ev1, s);
+        hipLaunchKernelGGL(f32::k_close, dim3(1), dim3(64), 0, s, k, a);
+    }
     return hipGetLastError();
 }
+
+int f32c_split_min_bare() { return 3000; }
 
 hipError_t f32c_recompute(const Consts& k, const float* W, const double* H, const double* dt_seq, int step, float* P,
                           float* ACC, hipStream_t s)

@@ -31,6 +31,13 @@ struct StepArgs {
     int hcap;
     hipEvent_t ev0, ev1;    // when set: recorded around the main step kernel only
     double t_end;           // fused kernels: a step launched at t >= t_end does nothing
+    // FP32C fused step by tile class: tiles whose tile + halo box holds
+    // sediment at step s (tcls[s & 1][t] == s, listed in tlist[s & 1][0 ..
+    // ctl->gcount[s & 1])) take the general tile function, the others the
+    // sediment-free one
+    int* tcls;              // [2][ntiles]
+    int* tlist;             // [2][ntiles]
+    int split;              // 1: sediment-free and general launches, 0: every tile general
 };
 
 // Device buffers of one simulation in FP64.
@@ -58,6 +65,12 @@ hipError_t f64_step_fused(const Consts& k, const double* W, double* Wn, const do
 hipError_t f32c_initial_dt(const Consts& k, const float* W, Ctl* ctl, hipStream_t s);
 hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const double* H, const StepArgs& a,
                            hipStream_t s);
+// tiles of the split step and their class table (tcls, tlist: 2 * f32c_tiles ints each)
+int f32c_tiles(const Consts& k);
+// fewest sediment-free tiles for which the split step pays
+int f32c_split_min_bare();
+// class table of state W at step ctl->step (after an upload or a restore)
+hipError_t f32c_classify(const Consts& k, const float* W, const StepArgs& a, hipStream_t s);
 // partial h sums read by classes 2..4 (P, [3][2][N]) and flux sums (ACC, [6][N])
 // of the step from W with dt = dt_seq[step], by the fused tile kernel
 hipError_t f32c_recompute(const Consts& k, const float* W, const double* H, const double* dt_seq, int step, float* P,

@@ -120,6 +120,8 @@ SIGNATURES = {
     "lhysea_dt_history": (C.c_int, [C.c_void_p, _dp, C.c_int32, _ip]),
     "lhysea_set_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
     "lhysea_time_steps": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
+    "lhysea_set_tile_split": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lhysea_tile_classes": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
     "lhysea_adjoint_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, C.c_int32, _dp]),
     "lhysea_adjoint_backward": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
     "lhysea_adjoint": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, C.c_int32, _dp, _dp, _dp, _dp, _dp,
@@ -276,6 +278,16 @@ class Simulation:
         n = C.c_int32()
         _check(self.lib.lhysea_dt_history(self.h, _dptr(out), max_steps, C.byref(n)))
         return out[: n.value]
+
+    def set_tile_split(self, mode: int):
+        """FP32C launch split by tile class: -1 automatic, 0 every tile general, 1 always split."""
+        _check(self.lib.lhysea_set_tile_split(self.h, mode))
+
+    def tile_classes(self):
+        """(class tiles, general tiles of the next step, last launches split)."""
+        t, g, sp = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(self.lib.lhysea_tile_classes(self.h, C.byref(t), C.byref(g), C.byref(sp)))
+        return t.value, g.value, bool(sp.value)
 
     def time_steps(self, nsteps: int, per_kernel: bool = True):
         tot, ker = C.c_double(), C.c_double()
