@@ -2,9 +2,11 @@
 
 Workload: the 2048x2048 precision-comparison case (plane shelf + 8 seeded
 Gaussian bumps, 4 seeded slides of 20-80 m, water at rest, walls, CFL 0.5,
-r 0.5, Coulomb 10 deg), forward steps of the fused step kernel.  A "step" is
-one solver time step over the whole grid: the four edge classes, the cell
-update and the on-device dt reduction.
+r 0.5, Coulomb 10 deg), forward steps of the fused step.  A "step" is one
+solver time step over the whole grid: the four edge classes, the cell update
+and the on-device dt reduction; in FP32C its launches are the sediment-free
+tiles' kernel and the general tiles' kernel, which also closes the step
+(tile_classes reports how many class tiles took the general one).
 
 Prints one JSON line.  value = Mcell-updates/s of the headline path (FP32
 with compensated state when available, else FP64) with the state resident in
@@ -33,11 +35,13 @@ from landslidehysea_amd import solver as L            # noqa: E402
 # 6x8 B; FP32c reads 6 values + 6 compensations (12x4 B) + 8 B depth and writes 12x4 B
 BYTES_PER_CELL = 104
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the fused FP32C step kernel at 2048x2048, measured with
-# rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the scale factors of a plain
-# 256 MiB copy (profiles/fused_step_fp32c_2048x2048_counters.txt,
-# profiles/copy_256MiB_counter_calibration.txt)
-TRAFFIC_BYTES_FP32C = 406.0e6
+# HBM bytes per FP32C step at 2048x2048 (the sediment-free and the general
+# tiles' launches together), measured with rocprofv3 FETCH_SIZE (x2.0) and
+# WRITE_SIZE (x1.0), the scale factors of a plain 256 MiB copy
+# (profiles/fused_step_fp32c_2048x2048_counters.txt,
+# profiles/copy_256MiB_counter_calibration.txt); below the algorithmic bytes
+# because sediment-free tiles do not read their zero layer-2 planes
+TRAFFIC_BYTES_FP32C = 353.4e6
 NX = NY = 2048
 SEED = 2048
 
@@ -62,9 +66,10 @@ def time_gpu(precision, steps, warmup):
         sim.sync()
         wall = time.perf_counter() - t0
         _, kernel_ms = sim.time_steps(max(steps // 2, 5), per_kernel=True)
+        classes = sim.tile_classes()
         t, dtn, n = sim.time()
     N = NX * NY
-    return {"total_ms": total_ms, "wall_s": wall, "kernel_ms": kernel_ms,
+    return {"total_ms": total_ms, "wall_s": wall, "kernel_ms": kernel_ms, "classes": classes,
             "mcells": N * steps / (total_ms * 1e-3) / 1e6, "t": t, "steps_done": n}
 
 
@@ -273,6 +278,9 @@ def main():
                      "traffic": TRAFFIC_BYTES_FP32C if head is fp32c else None,
                      "kernel_ms": round(head["kernel_ms"], 5), "bytes_per_cell": BYTES_PER_CELL},
     }
+    if fp32c:
+        nt, ngen, split = fp32c["classes"]
+        out["tile_classes"] = {"class_tiles": nt, "general_tiles": ngen, "split_launch": split}
     if args.adjoint_steps > 0:
         adj = {}
         if fp32c:
