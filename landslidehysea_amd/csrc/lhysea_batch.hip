@@ -42,7 +42,7 @@ int bhip_fail(hipError_t e, const char* where)
 
 // steps per captured graph; longer runs replay it, the remainder is launched directly
 constexpr int kGraphSteps = 16;
-constexpr int kMaxScen = 4096;          // scenario index field of the block table (12 bits)
+constexpr int kMaxScen = kMaxBatchScen; // scenario index field of the block table (12 bits)
 constexpr int kMaxTilesPerScen = 1 << 20;
 
 struct Scen {
@@ -60,6 +60,11 @@ struct Scen {
     bool track_emax = false;
     double t_end = INFINITY;
     int max_steps = INT_MAX;
+    // class tiles of the split shape (62 x 14): table and list per parity
+    int* cls = nullptr;
+    int* list = nullptr;
+    int ctx = 0, cty = 0;
+    bool cls_dirty = true;           // the class table does not describe the current state
 };
 
 }  // namespace
@@ -100,6 +105,10 @@ BatchScen descriptor(const lhysea_batch* b, int i, bool with_outputs)
     const BatchTile& t = kBatchTiles[b->tile];
     d.tiles_x = (s.p.nx + t.tx - 1) / t.tx;
     d.tiles_y = (s.p.ny + t.ty - 1) / t.ty;
+    d.cls = s.cls;
+    d.list = s.list;
+    d.ctiles_x = s.ctx;
+    d.ctiles_y = s.cty;
     if (with_outputs) {
         d.emax = s.track_emax ? s.emax : nullptr;
         d.gauges = s.gauges;
@@ -160,6 +169,17 @@ int prepare(lhysea_batch* b, bool with_outputs)
     if (b->tab_dirty && (rc = build_table(b)) != LHYSEA_OK) return rc;
     if ((b->sd_dirty || b->sd_outputs != with_outputs) && (rc = upload_descriptors(b, with_outputs)) != LHYSEA_OK)
         return rc;
+    if (b->tile == kBatchSplit) {
+        // class tables of scenarios whose state changed other than by a split step
+        for (size_t i = 0; i < b->sc.size(); i++) {
+            if (!b->sc[i].cls_dirty) continue;
+            BHIPCHK(f32c_batch_classify(descriptor(b, (int)i, with_outputs), b->d_sd, (int)i, b->stream));
+            b->sc[i].cls_dirty = false;
+        }
+    } else {
+        // the other shapes do not keep the class tables
+        for (Scen& sc : b->sc) sc.cls_dirty = true;
+    }
     return LHYSEA_OK;
 }
 
@@ -306,9 +326,14 @@ int lhysea_batch_create(const lhysea_params* p, int32_t n, lhysea_batch** out)
         for (int k = 0; k < 2; k++)
             if (hipMalloc(&s.W[k], 12 * s.N * sizeof(float)) != hipSuccess)
                 return cleanup(LHYSEA_ERR_GPU_MEMORY, "device state allocation failed");
+        s.ctx = (p[i].nx + kBatchTiles[kBatchSplit].tx - 1) / kBatchTiles[kBatchSplit].tx;
+        s.cty = (p[i].ny + kBatchTiles[kBatchSplit].ty - 1) / kBatchTiles[kBatchSplit].ty;
+        const size_t nct = (size_t)s.ctx * s.cty;
         if (hipMalloc(&s.H, s.N * sizeof(double)) != hipSuccess ||
             hipMalloc(&s.emax, 2 * s.N * sizeof(double)) != hipSuccess ||
-            hipMalloc(&s.dt_hist, (size_t)s.hcap * sizeof(double)) != hipSuccess)
+            hipMalloc(&s.dt_hist, (size_t)s.hcap * sizeof(double)) != hipSuccess ||
+            hipMalloc(&s.cls, 2 * nct * sizeof(int)) != hipSuccess ||
+            hipMalloc(&s.list, 2 * nct * sizeof(int)) != hipSuccess)
             return cleanup(LHYSEA_ERR_GPU_MEMORY, "device allocation failed");
     }
     *out = b;
@@ -328,6 +353,8 @@ int lhysea_batch_destroy(lhysea_batch* b)
         hipFree(s.gauges);
         hipFree(s.gseries);
         hipFree(s.dt_hist);
+        hipFree(s.cls);
+        hipFree(s.list);
     }
     hipFree(b->ctl);
     hipFree(b->d_sd);
@@ -371,6 +398,7 @@ int lhysea_batch_upload(lhysea_batch* b, int32_t sid, const double* W, const dou
     BHIPCHK(hipMemcpy(b->ctl + sid, &c, sizeof(Ctl), hipMemcpyHostToDevice));
     if (s.gstart != 0) b->sd_dirty = true;
     s.gstart = 0;
+    s.cls_dirty = true;
     return LHYSEA_OK;
 }
 
@@ -581,6 +609,7 @@ int lhysea_batch_autotune(lhysea_batch* b, int32_t nsteps, double* ms_per_tile, 
             BHIPCHK(hipMemcpy(s.W[c[i].step & 1], save[i], 12 * s.N * sizeof(float), hipMemcpyDeviceToDevice));
         }
         BHIPCHK(hipMemcpy(b->ctl, c.data(), (size_t)n * sizeof(Ctl), hipMemcpyHostToDevice));
+        for (Scen& sc : b->sc) sc.cls_dirty = true;
         return LHYSEA_OK;
     };
     const int keep = b->tile;

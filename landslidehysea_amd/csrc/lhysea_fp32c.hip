@@ -9,6 +9,7 @@
 // TX x TY tile with a 1-cell halo staged in LDS, the four edge classes in the
 // reference's order with LDS accumulators, the compensated cell update, and
 // a wavefront-shuffle max of the wave-speed sums for the lagged dt.
+#include <algorithm>
 #include "lhysea_internal.hpp"
 #include "lhysea_fp32.hpp"
 
@@ -990,6 +991,16 @@ __global__ void __launch_bounds__(NT) k_recompute(Consts k, const float* __restr
     tile_step<TX, TY, NT, true>(k, W, nullptr, H, dt_seq[step], 0.0, nullptr, tx * TX, ty * TY, P, ACC);
 }
 
+// A pointer read from a descriptor in memory is generic to the compiler, so
+// its accesses would be flat instructions (which also count against the LDS
+// wait counter); the batch buffers are all device memory, and passing them
+// through the global address space gives global loads and stores.
+template <typename T>
+__device__ __forceinline__ T* gmem(T* p)
+{
+    return (T*)(__attribute__((address_space(1))) T*)(uintptr_t)p;
+}
+
 // ------------------------------------------------------------- ensemble step
 // One launch steps every scenario of a batch.  tab[blockIdx.x] = (scenario
 // << 20) | tile, built on the host with the same XCD-contiguous order over the
@@ -1005,16 +1016,145 @@ __global__ void __launch_bounds__(NT) k_batch(const BatchScen* __restrict__ sd, 
 {
     const unsigned e = tab[blockIdx.x];
     const BatchScen& d = sd[e >> 20];
-    Ctl* ctl = d.ctl;
+    Ctl* ctl = gmem(d.ctl);
     const int step = ctl->step;
     if (!scen_active(d, step, ctl->t)) return;
     const int tile = (int)(e & 0xFFFFFu);
     const int tx = tile % d.tiles_x, ty = tile / d.tiles_x;
     const int cur = step & 1;
     const Consts k = d.k;
-    const double Amax = tile_step<TX, TY, NT>(k, d.W[cur], d.W[cur ^ 1], d.H, ctl->dt_cur, ctl->t, d.emax,
-                                              tx * TX, ty * TY);
+    const double Amax = tile_step<TX, TY, NT>(k, gmem(d.W[cur]), gmem(d.W[cur ^ 1]), gmem(d.H), ctl->dt_cur, ctl->t,
+                                              d.emax ? gmem(d.emax) : nullptr, tx * TX, ty * TY);
     block_max_atomic<NT>(Amax, &ctl->maxA_next);
+}
+
+// Split shape of the batch: the sediment-free class tiles of every scenario
+// (the block table lists all 62 x 14 class tiles; general ones leave), then
+// the general class tiles of every scenario cut into sub-tiles, each
+// scenario walked by its own group of gps workgroups.  Classes as in the
+// single-simulation step, per scenario.
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_batch_bare(const BatchScen* __restrict__ sd, const unsigned* __restrict__ tab)
+{
+    const unsigned e = tab[blockIdx.x];
+    const BatchScen& d = sd[e >> 20];
+    Ctl* ctl = gmem(d.ctl);
+    const int step = ctl->step;
+    if (!scen_active(d, step, ctl->t)) return;
+    const int tile = (int)(e & 0xFFFFFu);
+    const int tx = tile % d.ctiles_x, ty = tile / d.ctiles_x;
+    const int nct = d.ctiles_x * d.ctiles_y;
+    const int cur = step & 1;
+    const Consts k = d.k;
+    double Amax;
+    if (!tile_step_bare<TX, TY, NT>(k, gmem(d.W[cur]), gmem(d.W[cur ^ 1]), gmem(d.H), ctl->dt_cur, ctl->t,
+                                    d.emax ? gmem(d.emax) : nullptr, tx * TX, ty * TY,
+                                    gmem(d.cls) + (step & 1) * nct + tile, step, GaugeOut{nullptr, 0, nullptr}, Amax))
+        return;
+    block_max_atomic<NT>(Amax, &ctl->maxA_next);
+}
+
+// General sub-tiles of every scenario as one item space: item i of the
+// concatenation of the scenarios' lists (a scenario holds 4 x its general
+// tile count of items while active, none when stopped) goes to workgroup
+// i mod gridDim.x.  Each workgroup forms the prefix sums of the counts in LDS
+// and finds an item's scenario by binary search.
+template <int STX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_batch_gen(const BatchScen* __restrict__ sd, int nscen)
+{
+    constexpr int NSUB = (FTX + STX - 1) / STX;
+    static_assert(TY == FTY && STX % 2 == 0, "sub-tiles of a class tile");
+    __shared__ int pre[kMaxBatchScen + 1];
+    __shared__ int part[NT];
+    // per-thread run of scenarios, then the block scan of the run totals
+    const int per = (nscen + NT - 1) / NT, lo = threadIdx.x * per, hi = min(lo + per, nscen);
+    int run = 0;
+    for (int i = lo; i < hi; i++) {
+        const BatchScen& d = sd[i];
+        const int st = d.ctl->step;
+        const int c = scen_active(d, st, d.ctl->t) ? NSUB * (int)d.ctl->gcount[st & 1] : 0;
+        pre[i + 1] = c;
+        run += c;
+    }
+    part[threadIdx.x] = run;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int t = 0; t < NT; t++) {
+            const int v = part[t];
+            part[t] = acc;
+            acc += v;
+        }
+        pre[0] = 0;
+    }
+    __syncthreads();
+    {
+        int acc = part[threadIdx.x];
+        for (int i = lo; i < hi; i++) {
+            acc += pre[i + 1];
+            pre[i + 1] = acc;
+        }
+    }
+    __syncthreads();
+    const int total = pre[nscen];
+    double Amax = 0.0;
+    int sid_max = -1;
+    for (int i = blockIdx.x; i < total; i += gridDim.x) {
+        // scenario: the last sid with pre[sid] <= i
+        int l = 0, h = nscen - 1;
+        while (l < h) {
+            const int m = (l + h + 1) >> 1;
+            if (pre[m] <= i) l = m; else h = m - 1;
+        }
+        const int sid = l, item = i - pre[sid];
+        const BatchScen& d = sd[sid];
+        Ctl* ctl = gmem(d.ctl);
+        const int step = ctl->step;
+        const int nct = d.ctiles_x * d.ctiles_y;
+        const int t = gmem(d.list)[(step & 1) * nct + item / NSUB], sub = item % NSUB;
+        const int tx = t % d.ctiles_x, ty = t / d.ctiles_x;
+        const int cx0 = tx * FTX, cy0 = ty * FTY;
+        const int cur = step & 1;
+        int m = 0;
+        const double A = tile_step<STX, TY, NT>(d.k, gmem(d.W[cur]), gmem(d.W[cur ^ 1]), gmem(d.H), ctl->dt_cur,
+                                                ctl->t, d.emax ? gmem(d.emax) : nullptr, cx0 + sub * STX, cy0,
+                                                nullptr, nullptr, cx0 + FTX, GaugeOut{nullptr, 0, nullptr}, &m, cx0,
+                                                cy0);
+        m = block_or(m);   // also orders this sub-tile's LDS reads before the next one's staging
+        if (m && threadIdx.x < 64)
+            mark_general(TileCls{gmem(d.cls), gmem(d.list), d.ctiles_x, d.ctiles_y}, ctl, tx, ty, step + 1, m);
+        // the wave-speed max goes to the item's scenario (flushed when the scenario changes)
+        if (sid != sid_max && sid_max >= 0) {
+            block_max_atomic<NT>(Amax, &sd[sid_max].ctl->maxA_next);
+            Amax = 0.0;
+        }
+        sid_max = sid;
+        Amax = fmax(Amax, A);
+    }
+    if (sid_max >= 0) block_max_atomic<NT>(Amax, &sd[sid_max].ctl->maxA_next);
+}
+
+// Class table of one scenario at its current step: one block per class tile.
+__global__ void __launch_bounds__(256) k_batch_classify(const BatchScen* __restrict__ sd, int sid)
+{
+    const BatchScen& d = sd[sid];
+    const Consts& k = d.k;
+    const int nx = k.nx, ny = k.ny;
+    const size_t N = (size_t)nx * ny;
+    const int step = d.ctl->step;
+    const float* W = d.W[step & 1];
+    const int t = blockIdx.x, tx = t % d.ctiles_x, ty = t / d.ctiles_x;
+    int m = 0;
+    for (int e = threadIdx.x; e < FTX * FTY; e += 256) {
+        const int ly = e / FTX, lx = e % FTX;
+        const int gj = ty * FTY + ly, gi = tx * FTX + lx;
+        if (gj >= ny || gi >= nx) continue;
+        const size_t g = (size_t)gj * nx + gi;
+        if (W[3 * N + g] != 0.0f || W[9 * N + g] != 0.0f || W[4 * N + g] != 0.0f || W[5 * N + g] != 0.0f)
+            m |= halo_mask(lx, ly);
+    }
+    m = block_or(m);
+    if (m && threadIdx.x < 64) mark_general(TileCls{d.cls, d.list, d.ctiles_x, d.ctiles_y}, d.ctl, tx, ty, step, m);
 }
 
 // One block per scenario: gauges of the new state, dt history, t += dt, the
@@ -1022,7 +1162,7 @@ __global__ void __launch_bounds__(NT) k_batch(const BatchScen* __restrict__ sd, 
 __global__ void k_batch_finalize(const BatchScen* __restrict__ sd)
 {
     const BatchScen& d = sd[blockIdx.x];
-    Ctl* ctl = d.ctl;
+    Ctl* ctl = gmem(d.ctl);
     const int s = ctl->step;
     if (!scen_active(d, s, ctl->t)) return;
     const double dt = ctl->dt_cur;
@@ -1043,6 +1183,7 @@ __global__ void k_batch_finalize(const BatchScen* __restrict__ sd)
         ctl->dt_cur = dt_from_maxA(maxA, d.k.cfl, d.k.area);
         ctl->maxA_last = maxA;
         ctl->maxA_next = 0ull;
+        ctl->gcount[s & 1] = 0u;      // the general-tile list of this step is consumed
         ctl->step = s + 1;
     }
 }
@@ -1113,12 +1254,32 @@ hipError_t f32c_recompute(const Consts& k, const float* W, const double* H, cons
     return hipGetLastError();
 }
 
+// shape 0 (kBatchSplit) is the class-tile split: 62 x 14 class tiles of 512
+// threads for the sediment-free ones, general ones in 16 x 14 sub-tiles
 const BatchTile kBatchTiles[kNumBatchTiles] = {{62, 14, 512}, {30, 14, 256}, {62, 30, 1024}, {126, 14, 1024}};
+
+hipError_t f32c_batch_classify(const BatchScen& d, const BatchScen* sd_dev, int sid, hipStream_t s)
+{
+    const int nct = d.ctiles_x * d.ctiles_y;
+    hipError_t e = hipMemsetAsync(d.cls, 0xFF, 2 * (size_t)nct * sizeof(int), s);   // no tile general (-1)
+    if (e == hipSuccess) e = hipMemsetAsync(d.ctl->gcount, 0, sizeof(d.ctl->gcount), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(f32::k_batch_classify, dim3((unsigned)nct), dim3(256), 0, s, sd_dev, sid);
+    return hipGetLastError();
+}
 
 hipError_t f32c_batch_step(int tile, const BatchScen* sd, int nscen, const unsigned* tab, int nblocks, hipStream_t s)
 {
     switch (tile) {
-    case 0: hipLaunchKernelGGL((f32::k_batch<62, 14, 512>), dim3(nblocks), dim3(512), 0, s, sd, tab); break;
+    case kBatchSplit: {
+        // tab lists every 62 x 14 class tile; a fixed grid walks the general
+        // sub-tiles of all scenarios
+        hipLaunchKernelGGL((f32::k_batch_bare<f32::FTX, f32::FTY, f32::FNT>), dim3(nblocks), dim3(f32::FNT), 0, s, sd,
+                           tab);
+        hipLaunchKernelGGL((f32::k_batch_gen<f32::GSTX, f32::FTY, f32::GNT>), dim3(f32::GGRID), dim3(f32::GNT), 0, s,
+                           sd, nscen);
+        break;
+    }
     case 1: hipLaunchKernelGGL((f32::k_batch<30, 14, 256>), dim3(nblocks), dim3(256), 0, s, sd, tab); break;
     case 2: hipLaunchKernelGGL((f32::k_batch<62, 30, 1024>), dim3(nblocks), dim3(1024), 0, s, sd, tab); break;
     case 3: hipLaunchKernelGGL((f32::k_batch<126, 14, 1024>), dim3(nblocks), dim3(1024), 0, s, sd, tab); break;

@@ -89,6 +89,10 @@ struct BatchScen {
     double t_end;           // a scenario steps while t < t_end and step < max_steps
     int ng, gcap, hcap, max_steps;
     int tiles_x, tiles_y;
+    // class tiles (62 x 14) of the split shape: table and list per parity
+    int* cls;
+    int* list;
+    int ctiles_x, ctiles_y;
 };
 
 // Tile shapes of the batched kernel (TX x TY cells, NT threads)
@@ -99,6 +103,12 @@ constexpr int kNumBatchTiles = 4;
 extern const BatchTile kBatchTiles[kNumBatchTiles];
 hipError_t f32c_batch_step(int tile, const BatchScen* sd, int nscen, const unsigned* tab, int nblocks,
                            hipStream_t s);
+// scenarios of a batch at most
+constexpr int kMaxBatchScen = 4096;
+// shape kBatchSplit steps sediment-free class tiles and general ones apart
+constexpr int kBatchSplit = 0;
+// class table of scenario sid of the batch at its current step (device descriptors in sd)
+hipError_t f32c_batch_classify(const BatchScen& d, const BatchScen* sd_dev, int sid, hipStream_t s);
 
 // Adjoint (lhysea_adjoint.hip)
 hipError_t adj_reverse_step_f32(const Consts& k, const float* W, const double* H, float* P, float* ACC, float* AB,
