@@ -141,10 +141,12 @@ static void choose_split(lhysea_sim* s, const Ctl& c)
 
 // Tile classes of the current FP32C state at the current step; needed after
 // every change of the state or step count other than a step.
-static hipError_t classify(lhysea_sim* s)
+// With choose, the launch split is chosen from the new classes (a host sync).
+static hipError_t classify(lhysea_sim* s, bool choose)
 {
     if (s->p.precision != LHYSEA_FP32C) return hipSuccess;
     hipError_t e = f32c_classify(s->k, s->b32.W[s->cur], step_args(s), s->stream);
+    if (!choose) return e;
     Ctl c;
     if (e == hipSuccess) e = hipMemcpyAsync(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
@@ -327,7 +329,7 @@ int lhysea_upload(lhysea_sim* s, const double* W, const double* H)
     c.dt_cur = -1.0;
     HIPCHK(hipMemcpy(s->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice));
     s->gstart = 0;
-    HIPCHK(classify(s));
+    HIPCHK(classify(s, true));
     return LHYSEA_OK;
 }
 
@@ -671,7 +673,8 @@ hipError_t hook_advance(void* c, int k0, int n)
 {
     lhysea_adj_work* a = (lhysea_adj_work*)c;
     hipError_t e = adj_set_step(a->s->ctl, k0, a->s->stream);
-    if (e == hipSuccess) e = classify(a->s);   // the restored state's tile classes
+    // the restored state's tile classes (the launch split of the forward pass is kept)
+    if (e == hipSuccess) e = classify(a->s, false);
     if (e != hipSuccess) return e;
     return launch_steps(a->s, n) == LHYSEA_OK ? hipSuccess : hipErrorUnknown;
 }
