@@ -144,8 +144,8 @@ static void choose_split(lhysea_sim* s, const Ctl& c)
 // With choose, the launch split is chosen from the new classes (a host sync).
 static hipError_t classify(lhysea_sim* s, bool choose)
 {
-    if (s->p.precision != LHYSEA_FP32C) return hipSuccess;
-    hipError_t e = f32c_classify(s->k, s->b32.W[s->cur], step_args(s), s->stream);
+    hipError_t e = (s->p.precision == LHYSEA_FP32C) ? f32c_classify(s->k, s->b32.W[s->cur], step_args(s), s->stream)
+                                                     : f64_classify(s->k, s->b64.W[s->cur], step_args(s), s->stream);
     if (!choose) return e;
     Ctl c;
     if (e == hipSuccess) e = hipMemcpyAsync(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream);
@@ -247,19 +247,19 @@ int lhysea_create(const lhysea_params* p, lhysea_sim** out)
             hipMalloc(&s->b64.acc, 7 * N * sizeof(double)) != hipSuccess)
             return cleanup(LHYSEA_ERR_GPU_MEMORY, "device allocation failed");
         hipMemsetAsync(s->b64.acc, 0, 7 * N * sizeof(double), s->stream);
+        s->ntiles = f64_tiles(s->k);
     } else {
         for (int b = 0; b < 2; b++)
             if (hipMalloc(&s->b32.W[b], 12 * N * sizeof(float)) != hipSuccess)
                 return cleanup(LHYSEA_ERR_GPU_MEMORY, "device state allocation failed");
         if (hipMalloc(&s->b32.H, N * sizeof(double)) != hipSuccess)
             return cleanup(LHYSEA_ERR_GPU_MEMORY, "device allocation failed");
-        // tile classes of the step: two parities of stamps and lists
         s->ntiles = f32c_tiles(s->k);
-        const size_t nt = (size_t)s->ntiles;
-        if (hipMalloc(&s->tcls, 2 * nt * sizeof(int)) != hipSuccess ||
-            hipMalloc(&s->tlist, 2 * nt * sizeof(int)) != hipSuccess)
-            return cleanup(LHYSEA_ERR_GPU_MEMORY, "device allocation failed");
     }
+    // tile classes of the step: two parities of stamps and lists
+    if (hipMalloc(&s->tcls, 2 * (size_t)s->ntiles * sizeof(int)) != hipSuccess ||
+        hipMalloc(&s->tlist, 2 * (size_t)s->ntiles * sizeof(int)) != hipSuccess)
+        return cleanup(LHYSEA_ERR_GPU_MEMORY, "device allocation failed");
     if (hipMalloc(&s->ctl, sizeof(Ctl)) != hipSuccess || hipMalloc(&s->emax, 2 * N * sizeof(double)) != hipSuccess)
         return cleanup(LHYSEA_ERR_GPU_MEMORY, "device allocation failed");
     hipMemsetAsync(s->ctl, 0, sizeof(Ctl), s->stream);
@@ -535,6 +535,8 @@ int lhysea_set_kernel(lhysea_sim* s, int32_t mode)
     if (s->adj) return adj_busy();
     if (mode == MODE_FOURPASS && s->p.precision != LHYSEA_FP64)
         return fail(LHYSEA_ERR_ARGUMENT, "the four-pass kernels exist in FP64 only");
+    // the four-pass path keeps no tile classes: rebuild them on the way back
+    if (mode == MODE_FUSED && s->mode != MODE_FUSED) HIPCHK(classify(s, true));
     s->mode = mode;
     return LHYSEA_OK;
 }
@@ -554,7 +556,7 @@ int lhysea_tile_classes(lhysea_sim* s, int32_t* tiles, int32_t* general, int32_t
     HIPCHK(hipMemcpyAsync(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     if (tiles) *tiles = s->ntiles;
-    if (general) *general = (s->p.precision == LHYSEA_FP32C) ? (int32_t)c.gcount[c.step & 1] : 0;
+    if (general) *general = (int32_t)c.gcount[c.step & 1];
     if (split) *split = s->split;
     return LHYSEA_OK;
 }

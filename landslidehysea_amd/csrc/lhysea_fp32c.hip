@@ -12,6 +12,7 @@
 #include <algorithm>
 #include "lhysea_internal.hpp"
 #include "lhysea_fp32.hpp"
+#include "lhysea_tiles.hpp"
 
 namespace lhysea {
 namespace f32 {
@@ -155,43 +156,12 @@ __device__ __forceinline__ void accumulate(float (&sA)[7][TY][CW], int r, int c,
     sA[6][r][c] += lam;
 }
 
-// Class tiles whose tile + halo box holds a cell at (lx, ly) of a FTX x FTY
-// class tile, as bits (dy + 1) * 3 + (dx + 1) for the neighbour at (dx, dy):
-// the tile itself, and the neighbours across the edges and corners the cell
-// lies on.
 constexpr int FTX = 62, FTY = 14, FNT = 512;
 // general sub-tiles: width, threads, and the fixed grid walking them (four
 // workgroups per CU of an MI355X)
 constexpr int GSTX = 16, GNT = 256, GGRID = 1024;
-__device__ __forceinline__ int halo_mask(int lx, int ly)
-{
-    const int x0 = (lx == 0) ? 0 : 1, x1 = (lx == FTX - 1) ? 2 : 1;
-    const int y0 = (ly == 0) ? 0 : 1, y1 = (ly == FTY - 1) ? 2 : 1;
-    int m = 0;
-    for (int y = y0; y <= y1; y++)
-        for (int x = x0; x <= x1; x++) m |= 1 << (3 * y + x);
-    return m;
-}
+__device__ __forceinline__ int halo_mask(int lx, int ly) { return halo_mask_of<FTX, FTY>(lx, ly); }
 
-// Gauges inside the tile (columns below xend): eta = h1 + h2 - H from the
-// new h1 (sh1), h2 (sh2, or zero when null) and the depth the block staged,
-// after a barrier that publishes the threads' new values.
-// LDS planes are row-major with row length cwp, the tile's cell (r, c) at
-// row r + 1, column c + 1.
-template <int TX, int TY, int NT>
-__device__ __forceinline__ void gauges_of_tile(const GaugeOut& go, const double* sh1, const double* sh2,
-                                               const double* sH, int cwp, int nx, int i0, int j0, int xend)
-{
-    __syncthreads();
-    for (int gi = threadIdx.x; gi < go.ng; gi += NT) {
-        const int cell = go.cells[gi];
-        const int r = cell / nx - j0, c = cell % nx - i0;
-        if (r >= 0 && r < TY && c >= 0 && c < TX && i0 + c < xend) {
-            const int at = (r + 1) * cwp + c + 1;
-            go.out[gi] = (sh1[at] + (sh2 ? sh2[at] : 0.0)) - sH[at];
-        }
-    }
-}
 
 // ----------------------------------------------------------------- fused step
 // One even-aligned TX x TY tile of one step (origin i0, j0): stage the tile
@@ -738,81 +708,8 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
 }
 
 // ------------------------------------------------------------ tile classes
-// The single-simulation step is split by tile class.  The class tiles are the
-// FTX x FTY tiles of the grid.  A class tile is general at step s when its
-// tile + halo box holds a cell with sediment in the state of step s: its own
-// cells, or the edge and corner cells of its neighbours that form its halo.
-// cls[(s & 1) * nt + t] == s marks class tile t general at step s and
-// list[(s & 1) * nt + i], i < ctl->gcount[s & 1], enumerates those tiles; the
-// sediment-free kernel skips them and the general kernel takes them.  A cell
-// that keeps sediment through a general update marks, for step s + 1 (the
-// other parity, so the running step's entries stay intact), its own tile
-// and the neighbours whose halo it lies in.  A sediment-free tile cannot gain
-// sediment in one step (no edge of its box carries any), so it marks nothing.
-struct TileCls {
-    int* cls;
-    int* list;
-    int tiles_x, tiles_y;
-};
-
-// Marks, for step s, the class tiles of the 3 x 3 neighbourhood of (tx, ty)
-// whose bit is set in mask (bit (dy + 1) * 3 + (dx + 1)).  Called by the
-// first 9 threads of a wavefront with the same mask, thread b taking bit b,
-// so the atomics of the nine tiles are in flight together.
-__device__ __forceinline__ void mark_general(const TileCls& tc, Ctl* ctl, int tx, int ty, int s, int mask)
-{
-    const int bit = threadIdx.x & 63;
-    if (bit >= 9 || !(mask & (1 << bit))) return;
-    const int nt = tc.tiles_x * tc.tiles_y, base = (s & 1) * nt;
-    const int x = tx + bit % 3 - 1, y = ty + bit / 3 - 1;
-    if (x < 0 || y < 0 || x >= tc.tiles_x || y >= tc.tiles_y) return;
-    const int t = y * tc.tiles_x + x;
-    // the first marker of t for step s appends it
-    if (atomicMax(&tc.cls[base + t], s) < s) tc.list[base + (int)atomicAdd(&ctl->gcount[s & 1], 1u)] = t;
-}
-
-// Workgroup end of a general tile: thread 0 issues the atomicMax of the
-// wave-speed max v and marks the next step's general tiles from the OR of
-// the halo masks m (one barrier for both reductions).
-template <int NT>
-__device__ __forceinline__ void general_tile_end(double v, int m, const TileCls& tc, Ctl* ctl, int tx, int ty,
-                                                 int s_next)
-{
-    __shared__ double red[NT / 64];
-    __shared__ int ored[NT / 64];
-    for (int off = 32; off > 0; off >>= 1) {
-        v = fmax(v, __shfl_xor(v, off, 64));
-        m |= __shfl_xor(m, off, 64);
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) {
-        red[wid] = v;
-        ored[wid] = m;
-    }
-    __syncthreads();
-    if (wid == 0) {
-        // every lane of wavefront 0 forms both reductions
-        for (int i = 1; i < NT / 64; i++) {
-            v = fmax(v, red[i]);
-            m |= ored[i];
-        }
-        if (lane == 0 && v > 0.0) atomicMax(&ctl->maxA_next, (unsigned long long)__double_as_longlong(v));
-        if (m) mark_general(tc, ctl, tx, ty, s_next, m);
-    }
-}
-
-// Block-wide OR of m; every thread calls it.
-__device__ __forceinline__ int block_or(int m)
-{
-    __shared__ int acc;
-    if (threadIdx.x == 0) acc = 0;
-    __syncthreads();
-    if (m) atomicOr(&acc, m);
-    __syncthreads();
-    const int r = acc;
-    __syncthreads();
-    return r;
-}
+// The single-simulation step is split by tile class (lhysea_tiles.hpp); the
+// class tiles are the FTX x FTY tiles of the grid.
 
 // Sediment-free tiles of a step: one block per class tile, tiles of one XCD
 // contiguous (blocks are dispatched round-robin over the 8 XCDs, so XCD x
@@ -1206,7 +1103,7 @@ int f32c_tiles(const Consts& k)
 hipError_t f32c_classify(const Consts& k, const float* W, const StepArgs& a, hipStream_t s)
 {
     const int nt = f32c_tiles(k);
-    const f32::TileCls tc{a.tcls, a.tlist, (k.nx + f32::FTX - 1) / f32::FTX, (k.ny + f32::FTY - 1) / f32::FTY};
+    const TileCls tc{a.tcls, a.tlist, (k.nx + f32::FTX - 1) / f32::FTX, (k.ny + f32::FTY - 1) / f32::FTY};
     hipError_t e = hipMemsetAsync(a.tcls, 0xFF, 2 * (size_t)nt * sizeof(int), s);   // no tile general (-1)
     if (e == hipSuccess) e = hipMemsetAsync(a.ctl->gcount, 0, sizeof(a.ctl->gcount), s);
     if (e != hipSuccess) return e;
@@ -1225,7 +1122,7 @@ hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const dou
     const int tiles_x = (k.nx + f32::FTX - 1) / f32::FTX;
     const int tiles_y = (k.ny + f32::FTY - 1) / f32::FTY;
     const int nt = tiles_x * tiles_y;
-    const f32::TileCls tc{a.tcls, a.tlist, tiles_x, tiles_y};
+    const TileCls tc{a.tcls, a.tlist, tiles_x, tiles_y};
     if (a.ev0) hipEventRecord(a.ev0, s);
     if (a.split) {
         hipLaunchKernelGGL((f32::k_fused_bare<f32::FTX, f32::FTY, f32::FNT>), dim3((unsigned)nt), dim3(f32::FNT), 0,

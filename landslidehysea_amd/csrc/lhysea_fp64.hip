@@ -17,6 +17,7 @@
 #pragma clang fp contract(off)
 #include "lhysea_internal.hpp"
 #include "lhysea_fp64.hpp"
+#include "lhysea_tiles.hpp"
 
 namespace lhysea {
 namespace f64 {
@@ -269,34 +270,26 @@ __device__ __forceinline__ void accumulate(double (&sA)[7][TY][CW], int r, int c
     sA[6][r][c] += lam;
 }
 
-// Tile of TX x TY cells (both even), NT threads.  LDS planes:
+// Tile of TX x TY cells (both even), NT threads, origin (i0, j0).  LDS planes:
 //   sW[f][r][c]  f = h1,q1x,q1y,h2,q2x,q2y,H ; r = tile row + 1 ; c = tile col + 1
 //   sA[f][r][c]  f = dh1..dq2y, A            ; r = tile row     ; c = tile col + 1
+// Returns this thread's max of the wave-speed sums of the cells it updated.
+// go: gauges of the new state.  smask (optional): OR of the class-tile halo
+// masks of this thread's cells that keep sediment, the class tile at
+// (cx0, cy0).
 template <int TX, int TY, int NT>
-__global__ void __launch_bounds__(NT) k_fused(Consts k, const double* __restrict__ W, double* __restrict__ Wn,
-                                              const double* __restrict__ H, StepArgs a, int tiles_x, int tiles_y)
+__device__ __forceinline__ double tile_step64(const Consts& k, const double* __restrict__ W, double* __restrict__ Wn,
+                                              const double* __restrict__ H, double dt, double tnow,
+                                              double* __restrict__ emax, int i0, int j0, GaugeOut go, int* smask,
+                                              int cx0, int cy0)
 {
     constexpr int CW = TX + 2;              // staged columns
     constexpr int RW = TY + 2;              // staged rows
     __shared__ double sW[7][RW][CW];
     __shared__ double sA[7][TY][CW];
 
-    if (!(a.ctl->t < a.t_end)) return;     // the run has reached its end time
     const int nx = k.nx, ny = k.ny;
     const size_t N = (size_t)nx * ny;
-    // XCD-aware tile order: consecutive workgroups go round-robin over the 8
-    // XCDs, so give each XCD a contiguous band of tiles (shared halos stay in
-    // that XCD's L2).
-    const int nb = tiles_x * tiles_y;
-    int b;
-    {
-        const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-        const int q = nb >> 3, rem = nb & 7;
-        b = xcd * q + (xcd < rem ? xcd : rem) + slot;
-    }
-    const int tx = b % tiles_x, ty = b / tiles_x;
-    const int i0 = tx * TX, j0 = ty * TY;
-    const double dt = step_dt(a.ctl, a.dt_seq);
     const int tid = threadIdx.x;
 
     // stage state + depth with a 1-cell halo; zero the accumulators
@@ -421,7 +414,6 @@ __global__ void __launch_bounds__(NT) k_fused(Consts k, const double* __restrict
 
     // cell update of the tile's own cells
     double Amax = 0.0;
-    const double tnow = a.ctl->t;
     for (int e = tid; e < TX * TY; e += NT) {
         const int r = e / TX, c = e % TX;
         const int gj = j0 + r, gi = i0 + c;
@@ -431,11 +423,11 @@ __global__ void __launch_bounds__(NT) k_fused(Consts k, const double* __restrict
 #pragma unroll
         for (int f = 0; f < 6; f++) { o[f] = sW[f][r + 1][c + 1]; ac[f] = sA[f][r][c + 1]; }
         Amax = fmax(Amax, sA[6][r][c + 1]);
-        if (a.emax) {
+        if (emax) {
             const double v = o[0] - sW[6][r + 1][c + 1];
-            if (v > a.emax[2 * g]) {
-                a.emax[2 * g] = v;
-                a.emax[2 * g + 1] = tnow;
+            if (v > emax[2 * g]) {
+                emax[2 * g] = v;
+                emax[2 * g + 1] = tnow;
             }
         }
         // a wavefront of cells without sediment and layer-2 sums takes the
@@ -448,11 +440,316 @@ __global__ void __launch_bounds__(NT) k_fused(Consts k, const double* __restrict
             cell_update(k, o, ac, dt, n);
 #pragma unroll
         for (int f = 0; f < 6; f++) Wn[f * N + g] = n[f];
+        if (go.out) {
+            // the new h1, h2 replace this cell's staged ones (no other thread reads them now)
+            sW[0][r + 1][c + 1] = n[0];
+            sW[3][r + 1][c + 1] = n[3];
+        }
+        if (smask && (n[3] != 0.0 || n[4] != 0.0 || n[5] != 0.0)) *smask |= halo_mask_of<TX, TY>(gi - cx0, gj - cy0);
     }
-    block_max_atomic<NT>(Amax, &a.ctl->maxA_next);
+    if (go.out) gauges_of_tile<TX, TY, NT>(go, &sW[0][0][0], &sW[3][0][0], &sW[6][0][0], CW, nx, i0, j0, 0x7fffffff);
+    return Amax;
 }
 
 constexpr int FTX = 30, FTY = 14, FNT = 256;
+// general tiles walked by a fixed grid (four workgroups per CU of an MI355X)
+constexpr int GGRID = 1024;
+
+// tile_step64() for a tile whose cells and halo carry no sediment (h2 = q2 =
+// 0 in every staged cell): every edge takes edge1() and every cell
+// cell_update1(), the arithmetic tile_step64() performs on such a tile, so
+// the results are the same; the layer-2 planes are neither read nor kept in
+// LDS (zeros are written).  A tile whose class entry *cls equals s belongs to
+// the general kernel: the block returns false before its first LDS store.
+template <int TX, int TY, int NT>
+__device__ __forceinline__ bool tile_step_bare64(const Consts& k, const double* __restrict__ W,
+                                                 double* __restrict__ Wn, const double* __restrict__ H, double dt,
+                                                 double tnow, double* __restrict__ emax, int i0, int j0,
+                                                 const int* cls, int s, GaugeOut go, double& Amax)
+{
+    constexpr int CW = TX + 2, RW = TY + 2;
+    __shared__ double sW[4][RW][CW];   // h1, q1x, q1y, H
+    __shared__ double sA[4][TY][CW];   // dh1, dq1x, dq1y, wave-speed sum
+    const int nx = k.nx, ny = k.ny;
+    const size_t N = (size_t)nx * ny;
+    const int tid = threadIdx.x;
+
+    constexpr int ST_IT = (RW * CW + NT - 1) / NT;
+    double v[ST_IT][4];
+    bool in[ST_IT];
+#pragma unroll
+    for (int it = 0; it < ST_IT; it++) {
+        const int e = tid + it * NT;
+        const int r = e / CW, c = e % CW;
+        const int gj = j0 + r - 1, gi = i0 + c - 1;
+        in[it] = e < RW * CW && gj >= 0 && gj < ny && gi >= 0 && gi < nx;
+        if (in[it]) {
+            const size_t g = (size_t)gj * nx + gi;
+            v[it][0] = W[g];
+            v[it][1] = W[N + g];
+            v[it][2] = W[2 * N + g];
+            v[it][3] = H[g];
+        }
+    }
+    if (*cls == s) return false;
+#pragma unroll
+    for (int it = 0; it < ST_IT; it++) {
+        if (!in[it]) continue;
+        const int e = tid + it * NT;
+        const int r = e / CW, c = e % CW;
+#pragma unroll
+        for (int f = 0; f < 4; f++) sW[f][r][c] = v[it][f];
+    }
+    for (int e = tid; e < TY * CW; e += NT) {
+        const int r = e / CW, c = e % CW;
+#pragma unroll
+        for (int f = 0; f < 4; f++) sA[f][r][c] = 0.0;
+    }
+    __syncthreads();
+    auto acc = [&](int r, int c, const double* F, double lam) {
+#pragma unroll
+        for (int f = 0; f < 3; f++) sA[f][r][c] -= F[f];
+        sA[3][r][c] += lam;
+    };
+
+    // y-edge classes C1 then C2
+#pragma unroll 1
+    for (int par = 0; par < 2; par++) {
+        const int nrow = (par == 0) ? (TY / 2 + 1) : (TY / 2);
+        for (int e = tid; e < nrow * CW; e += NT) {
+            const int c = e % CW, jl = par + 2 * (e / CW);
+            const int gi = i0 + c - 1, je = j0 + jl;
+            if (gi < 0 || gi >= nx || je > ny) continue;
+            double W0[3], W1[3], H0, H1, a1h1 = 0.0, n1y;
+            int r0;
+            bool interior;
+            if (je == 0 || je == ny) {
+                r0 = (je == 0) ? jl : jl - 1;
+                const double bd = (je == 0) ? k.borde_sup : k.borde_inf;
+#pragma unroll
+                for (int f = 0; f < 3; f++) { W0[f] = sW[f][r0 + 1][c]; W1[f] = W0[f]; }
+                W1[2] *= bd;
+                H0 = H1 = sW[3][r0 + 1][c];
+                n1y = (je == 0) ? -1.0 : 1.0;
+                interior = false;
+            } else {
+                r0 = jl - 1;
+#pragma unroll
+                for (int f = 0; f < 3; f++) { W0[f] = sW[f][r0 + 1][c]; W1[f] = sW[f][r0 + 2][c]; }
+                H0 = sW[3][r0 + 1][c];
+                H1 = sW[3][r0 + 2][c];
+                n1y = 1.0;
+                interior = true;
+            }
+            const bool has0 = (r0 >= 0 && r0 < TY), has1 = interior && (r0 + 1 < TY);
+            const double a0h1 = has0 ? sA[0][r0][c] : 0.0;
+            if (has1) a1h1 = sA[0][r0 + 1][c];
+            EdgeFlux F;
+            if (edge1(k, W0, W1, H0, H1, 0.0, n1y, k.dx, dt, a0h1, a1h1, interior, F)) {
+                if (has0) acc(r0, c, F.Fm, F.lam);
+                if (has1) acc(r0 + 1, c, F.Fp, F.lam);
+            }
+        }
+        __syncthreads();
+    }
+
+    // x-edge classes C3 then C4
+#pragma unroll 1
+    for (int par = 0; par < 2; par++) {
+        const int ncol = (par == 0) ? (TX / 2 + 1) : (TX / 2);
+        for (int e = tid; e < ncol * TY; e += NT) {
+            const int r = e / ncol, il = par + 2 * (e % ncol);
+            const int gj = j0 + r, ie = i0 + il;
+            if (gj >= ny || ie > nx) continue;
+            double W0[3], W1[3], H0, H1, a1h1 = 0.0, n1x;
+            int c0;
+            bool interior;
+            if (ie == 0 || ie == nx) {
+                c0 = (ie == 0) ? il + 1 : il;
+                const double bd = (ie == 0) ? k.borde_izq : k.borde_der;
+#pragma unroll
+                for (int f = 0; f < 3; f++) { W0[f] = sW[f][r + 1][c0]; W1[f] = W0[f]; }
+                W1[1] *= bd;
+                H0 = H1 = sW[3][r + 1][c0];
+                n1x = (ie == 0) ? -1.0 : 1.0;
+                interior = false;
+            } else {
+                c0 = il;
+#pragma unroll
+                for (int f = 0; f < 3; f++) { W0[f] = sW[f][r + 1][c0]; W1[f] = sW[f][r + 1][c0 + 1]; }
+                H0 = sW[3][r + 1][c0];
+                H1 = sW[3][r + 1][c0 + 1];
+                n1x = 1.0;
+                interior = true;
+            }
+            const double a0h1 = sA[0][r][c0];
+            if (interior) a1h1 = sA[0][r][c0 + 1];
+            EdgeFlux F;
+            if (edge1(k, W0, W1, H0, H1, n1x, 0.0, k.dy, dt, a0h1, a1h1, interior, F)) {
+                acc(r, c0, F.Fm, F.lam);
+                if (interior) acc(r, c0 + 1, F.Fp, F.lam);
+            }
+        }
+        __syncthreads();
+    }
+
+    // cell update (layer 2 stays zero)
+    Amax = 0.0;
+    for (int e = tid; e < TX * TY; e += NT) {
+        const int r = e / TX, c = e % TX;
+        const int gj = j0 + r, gi = i0 + c;
+        if (gj >= ny || gi >= nx) continue;
+        const size_t g = (size_t)gj * nx + gi;
+        double o[6] = {sW[0][r + 1][c + 1], sW[1][r + 1][c + 1], sW[2][r + 1][c + 1], 0.0, 0.0, 0.0};
+        double ac[6] = {sA[0][r][c + 1], sA[1][r][c + 1], sA[2][r][c + 1], 0.0, 0.0, 0.0};
+        double n[6];
+        Amax = fmax(Amax, sA[3][r][c + 1]);
+        if (emax) {
+            const double vv = o[0] - sW[3][r + 1][c + 1];
+            if (vv > emax[2 * g]) {
+                emax[2 * g] = vv;
+                emax[2 * g + 1] = tnow;
+            }
+        }
+        cell_update1(k, o, ac, dt, n);
+#pragma unroll
+        for (int f = 0; f < 6; f++) Wn[f * N + g] = n[f];
+        if (go.out) sW[0][r + 1][c + 1] = n[0];   // the new h1 (read by no other thread now)
+    }
+    if (go.out) gauges_of_tile<TX, TY, NT>(go, &sW[0][0][0], nullptr, &sW[3][0][0], CW, nx, i0, j0, 0x7fffffff);
+    return true;
+}
+
+__device__ __forceinline__ int xcd_tile(int nb)
+{
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int q = nb >> 3, rem = nb & 7;
+    return xcd * q + (xcd < rem ? xcd : rem) + slot;
+}
+
+// Every tile through the general tile function (the classes kept), one
+// block per tile in XCD-contiguous order (consecutive workgroups go
+// round-robin over the 8 XCDs, so each XCD gets a contiguous band of tiles
+// and shared halos stay in its L2).
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_fused_all(Consts k, const double* __restrict__ W, double* __restrict__ Wn,
+                                                  const double* __restrict__ H, StepArgs a, TileCls tc)
+{
+    if (!(a.ctl->t < a.t_end)) return;     // the run has reached its end time
+    const int b = xcd_tile(tc.tiles_x * tc.tiles_y);
+    const int tx = b % tc.tiles_x, ty = b / tc.tiles_x;
+    const int s = a.ctl->step;
+    const GaugeOut go{a.gauges, a.ng, (a.gseries && s < a.gcap) ? a.gseries + (size_t)s * a.ng : nullptr};
+    int m = 0;
+    const double Amax = tile_step64<TX, TY, NT>(k, W, Wn, H, step_dt(a.ctl, a.dt_seq), a.ctl->t, a.emax, tx * TX,
+                                                ty * TY, go, &m, tx * TX, ty * TY);
+    general_tile_end<NT>(Amax, m, tc, a.ctl, tx, ty, s + 1);
+}
+
+// Sediment-free tiles of a step, one block per tile (general ones leave).
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const double* __restrict__ W, double* __restrict__ Wn,
+                                                   const double* __restrict__ H, StepArgs a, TileCls tc)
+{
+    if (!(a.ctl->t < a.t_end)) return;
+    const int nb = tc.tiles_x * tc.tiles_y;
+    const int b = xcd_tile(nb);
+    const int tx = b % tc.tiles_x, ty = b / tc.tiles_x;
+    const int s = a.ctl->step;
+    const GaugeOut go{a.gauges, a.ng, (a.gseries && s < a.gcap) ? a.gseries + (size_t)s * a.ng : nullptr};
+    double Amax;
+    if (!tile_step_bare64<TX, TY, NT>(k, W, Wn, H, step_dt(a.ctl, a.dt_seq), a.ctl->t, a.emax, tx * TX, ty * TY,
+                                      tc.cls + (s & 1) * nb + b, s, go, Amax))
+        return;
+    block_max_atomic<NT>(Amax, &a.ctl->maxA_next);
+}
+
+// General tiles of a step (a fixed grid walks the list), then the step's
+// close by the last workgroup (src/GPU/ShallowWater.cu:431-442), as in the
+// FP32C general launch: its count increment uses the value its atomicMax
+// returned, so it is issued once that max has been performed.
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_fused_gen(Consts k, const double* __restrict__ W, double* __restrict__ Wn,
+                                                  const double* __restrict__ H, StepArgs a, TileCls tc)
+{
+    Ctl* ctl = a.ctl;
+    if (!(ctl->t < a.t_end)) return;
+    const int s = ctl->step;
+    const int n = (int)ctl->gcount[s & 1];
+    const unsigned part = (unsigned)max(min(n, (int)gridDim.x), 1);
+    if (blockIdx.x >= part) return;
+    const int nt = tc.tiles_x * tc.tiles_y;
+    const double dt = step_dt(ctl, a.dt_seq);
+    const double tnow = ctl->t;
+    const GaugeOut go{a.gauges, a.ng, (a.gseries && s < a.gcap) ? a.gseries + (size_t)s * a.ng : nullptr};
+    double Amax = 0.0;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int t = tc.list[(s & 1) * nt + i];
+        const int tx = t % tc.tiles_x, ty = t / tc.tiles_x;
+        int m = 0;
+        Amax = fmax(Amax, tile_step64<TX, TY, NT>(k, W, Wn, H, dt, tnow, a.emax, tx * TX, ty * TY, go, &m, tx * TX,
+                                                  ty * TY));
+        m = block_or(m);   // also orders this tile's LDS reads before the next one's staging
+        if (m && threadIdx.x < 64) mark_general(tc, ctl, tx, ty, s + 1, m);
+    }
+    __shared__ double red[NT / 64];
+    double v = Amax;
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NT / 64; w++) v = fmax(v, red[w]);
+        const unsigned long long old = atomicMax(&ctl->maxA_next, (unsigned long long)__double_as_longlong(v));
+        // bit 63 of a non-negative double is clear: the increment is 1
+        const unsigned done = atomicAdd(&ctl->blocks_done, 1u + (unsigned)(old >> 63));
+        if (done == part - 1) {
+            const double maxA = __longlong_as_double((long long)atomicAdd(&ctl->maxA_next, 0ull));
+            if (a.dt_hist && s < a.hcap) a.dt_hist[s] = dt;
+            ctl->t = tnow + dt;
+            ctl->dt_cur = dt_from_maxA(maxA, k.cfl, k.area);
+            ctl->maxA_last = maxA;
+            ctl->gcount[s & 1] = 0u;
+            atomicExch(&ctl->maxA_next, 0ull);
+            atomicExch(&ctl->blocks_done, 0u);
+            ctl->step = s + 1;
+        }
+    }
+}
+
+// Close of a step run by k_fused_all (src/GPU/ShallowWater.cu:431-442).
+__global__ void k_close(Consts k, StepArgs a)
+{
+    Ctl* ctl = a.ctl;
+    if (threadIdx.x != 0 || !(ctl->t < a.t_end)) return;
+    const int s = ctl->step;
+    const double dt = step_dt(ctl, a.dt_seq);
+    if (a.dt_hist && s < a.hcap) a.dt_hist[s] = dt;
+    const double maxA = __longlong_as_double((long long)ctl->maxA_next);
+    ctl->t = ctl->t + dt;
+    ctl->dt_cur = dt_from_maxA(maxA, k.cfl, k.area);
+    ctl->maxA_last = maxA;
+    ctl->maxA_next = 0ull;
+    ctl->gcount[s & 1] = 0u;
+    ctl->step = s + 1;
+}
+
+// Class table of the state W at step ctl->step: one block per class tile.
+__global__ void __launch_bounds__(256) k_classify(Consts k, const double* __restrict__ W, StepArgs a, TileCls tc)
+{
+    const int nx = k.nx, ny = k.ny;
+    const size_t N = (size_t)nx * ny;
+    const int t = blockIdx.x, tx = t % tc.tiles_x, ty = t / tc.tiles_x;
+    int m = 0;
+    for (int e = threadIdx.x; e < FTX * FTY; e += 256) {
+        const int ly = e / FTX, lx = e % FTX;
+        const int gj = ty * FTY + ly, gi = tx * FTX + lx;
+        if (gj >= ny || gi >= nx) continue;
+        const size_t g = (size_t)gj * nx + gi;
+        if (W[3 * N + g] != 0.0 || W[4 * N + g] != 0.0 || W[5 * N + g] != 0.0) m |= halo_mask_of<FTX, FTY>(lx, ly);
+    }
+    m = block_or(m);
+    if (m && threadIdx.x < 64) mark_general(tc, a.ctl, tx, ty, a.ctl->step, m);
+}
 
 }  // namespace f64
 
@@ -481,16 +778,44 @@ hipError_t f64_step_fourpass(const Consts& k, const double* W, double* Wn, const
     return hipGetLastError();
 }
 
+int f64_tiles(const Consts& k)
+{
+    return ((k.nx + f64::FTX - 1) / f64::FTX) * ((k.ny + f64::FTY - 1) / f64::FTY);
+}
+
+hipError_t f64_classify(const Consts& k, const double* W, const StepArgs& a, hipStream_t s)
+{
+    const int nt = f64_tiles(k);
+    const TileCls tc{a.tcls, a.tlist, (k.nx + f64::FTX - 1) / f64::FTX, (k.ny + f64::FTY - 1) / f64::FTY};
+    hipError_t e = hipMemsetAsync(a.tcls, 0xFF, 2 * (size_t)nt * sizeof(int), s);   // no tile general (-1)
+    if (e == hipSuccess) e = hipMemsetAsync(a.ctl->gcount, 0, sizeof(a.ctl->gcount), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(f64::k_classify, dim3((unsigned)nt), dim3(256), 0, s, k, W, a, tc);
+    return hipGetLastError();
+}
+
+// Split (a.split): the sediment-free launch, then the general tiles and the
+// step's close; otherwise every tile general, then the close.
 hipError_t f64_step_fused(const Consts& k, const double* W, double* Wn, const double* H, const StepArgs& a,
                           hipStream_t s)
 {
     const int tiles_x = (k.nx + f64::FTX - 1) / f64::FTX;
     const int tiles_y = (k.ny + f64::FTY - 1) / f64::FTY;
+    const int nt = tiles_x * tiles_y;
+    const TileCls tc{a.tcls, a.tlist, tiles_x, tiles_y};
     if (a.ev0) hipEventRecord(a.ev0, s);
-    hipLaunchKernelGGL((f64::k_fused<f64::FTX, f64::FTY, f64::FNT>), dim3((unsigned)(tiles_x * tiles_y)),
-                       dim3(f64::FNT), 0, s, k, W, Wn, H, a, tiles_x, tiles_y);
-    if (a.ev1) hipEventRecord(a.ev1, s);
-    hipLaunchKernelGGL(f64::k_finalize, dim3(1), dim3(64), 0, s, k, Wn, H, a);
+    if (a.split) {
+        hipLaunchKernelGGL((f64::k_fused_bare<f64::FTX, f64::FTY, f64::FNT>), dim3((unsigned)nt), dim3(f64::FNT), 0,
+                           s, k, W, Wn, H, a, tc);
+        hipLaunchKernelGGL((f64::k_fused_gen<f64::FTX, f64::FTY, f64::FNT>), dim3(f64::GGRID), dim3(f64::FNT), 0, s,
+                           k, W, Wn, H, a, tc);
+        if (a.ev1) hipEventRecord(a.ev1, s);
+    } else {
+        hipLaunchKernelGGL((f64::k_fused_all<f64::FTX, f64::FTY, f64::FNT>), dim3((unsigned)nt), dim3(f64::FNT), 0,
+                           s, k, W, Wn, H, a, tc);
+        if (a.ev1) hipEventRecord(a.ev1, s);
+        hipLaunchKernelGGL(f64::k_close, dim3(1), dim3(64), 0, s, k, a);
+    }
     return hipGetLastError();
 }
 

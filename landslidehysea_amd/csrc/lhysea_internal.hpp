@@ -60,6 +60,9 @@ hipError_t f64_step_fourpass(const Consts& k, const double* W, double* Wn, const
                              const StepArgs& a, hipStream_t s);
 hipError_t f64_step_fused(const Consts& k, const double* W, double* Wn, const double* H, const StepArgs& a,
                           hipStream_t s);
+// tiles of the FP64 split step and its class table (tcls, tlist: 2 * f64_tiles ints each)
+int f64_tiles(const Consts& k);
+hipError_t f64_classify(const Consts& k, const double* W, const StepArgs& a, hipStream_t s);
 
 // FP32-compensated launchers (lhysea_fp32c.hip)
 hipError_t f32c_initial_dt(const Consts& k, const float* W, Ctl* ctl, hipStream_t s);

@@ -91,3 +91,33 @@ def test_split_step_without_sediment_and_all_sediment():
             assert g0 == (nt if general is None else general)
             assert np.array_equal(a["W"], b["W"]) and a["time"] == b["time"]
             assert np.array_equal(a["gauges"], b["gauges"])
+
+
+@pytest.mark.parametrize("nx,ny,seed,angle,steps", [(400, 300, 5, 1.0, 160), (1024, 512, 7, 1.0, 40)])
+def test_fp64_split_step_equals_fourpass(nx, ny, seed, angle, steps):
+    """FP64: the split launch (forced on and off) equals the four-pass kernels
+    (the reference's structure, HBM accumulators) bit for bit."""
+    sc, W, H, prm = case(nx, ny, seed=seed, angle=angle, n_slides=3, semi_axes=(14.0, 9.0), thickness=(30.0, 60.0),
+                         precision=L.FP64, borde=(1.0, -1.0, 1.0, -1.0))
+    gauges = gauge_cells(sc, 8, seed=seed)
+
+    def run(kernel, split):
+        with L.Simulation(prm, kernel=kernel) as sim:
+            if split is not None:
+                sim.set_tile_split(split)
+            sim.upload(W, H)
+            sim.track_eta_max(True)
+            sim.set_gauges(gauges, steps)
+            sim.step(steps)
+            return {"W": sim.download(), "time": sim.time(), "dt": sim.dt_history(steps), "gauges": sim.gauges(),
+                    "emax": sim.eta_max(), "classes": sim.tile_classes()}
+
+    ref = run(L.KERNEL_FOURPASS, None)
+    for split in (1, 0):
+        a = run(L.KERNEL_FUSED, split)
+        nt, g, sp = a["classes"]
+        say(f"fp64 {nx}x{ny} split={split}: general tiles {g} of {nt}, launches split {sp}")
+        assert 0 < g < nt and sp == bool(split)
+        assert np.array_equal(a["W"], ref["W"]) and a["time"] == ref["time"]
+        assert np.array_equal(a["dt"], ref["dt"]) and np.array_equal(a["gauges"], ref["gauges"])
+        assert np.array_equal(a["emax"][0], ref["emax"][0]) and np.array_equal(a["emax"][1], ref["emax"][1])
