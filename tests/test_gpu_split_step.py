@@ -121,3 +121,23 @@ def test_fp64_split_step_equals_fourpass(nx, ny, seed, angle, steps):
         assert np.array_equal(a["W"], ref["W"]) and a["time"] == ref["time"]
         assert np.array_equal(a["dt"], ref["dt"]) and np.array_equal(a["gauges"], ref["gauges"])
         assert np.array_equal(a["emax"][0], ref["emax"][0]) and np.array_equal(a["emax"][1], ref["emax"][1])
+
+
+@pytest.mark.parametrize("precision", [L.FP32C, L.FP64])
+def test_split_step_run_until(precision):
+    """run_until (device-side stop inside launched chunks) with the split launch
+    forced on equals the all-tiles launch bit for bit, in both precisions."""
+    sc, W, H, prm = case(400, 300, seed=9, angle=1.0, n_slides=2, semi_axes=(14.0, 9.0), thickness=(30.0, 60.0),
+                         precision=precision)
+    out = {}
+    for split in (1, 0):
+        with L.Simulation(prm) as sim:
+            sim.set_tile_split(split)
+            sim.upload(W, H)
+            n1 = sim.run_until(2.5)
+            n2 = sim.run_until(4.0, 37)
+            out[split] = (sim.download(), sim.time(), n1, n2)
+    a, b = out[1], out[0]
+    say("run_until steps", a[2], a[3], "time", a[1])
+    assert a[2] == b[2] and a[3] == b[3] and a[1] == b[1]
+    assert np.array_equal(a[0], b[0])
