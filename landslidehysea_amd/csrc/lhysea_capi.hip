@@ -68,6 +68,7 @@ struct lhysea_sim {
     int ntiles = 0;
     int split = 0;
     int split_mode = -1;         // -1 chosen per launch piece, 0 never split, 1 always
+    int step_host = 0;           // ctl->step when the next launched step starts
 };
 
 
@@ -135,6 +136,7 @@ static StepArgs step_args(lhysea_sim* s)
 // sediment-free launch pays when enough tiles are sediment-free.
 static void choose_split(lhysea_sim* s, const Ctl& c)
 {
+    s->step_host = c.step;
     s->split = (s->split_mode >= 0) ? s->split_mode
                                     : (s->ntiles - (int)c.gcount[c.step & 1]) >= f32c_split_min_bare();
 }
@@ -169,8 +171,10 @@ static StepArgs launch_args(lhysea_sim* s, hipEvent_t ev0, hipEvent_t ev1)
 }
 
 // One step from state buffer W to Wn (either precision's layout).
-static hipError_t launch_one(lhysea_sim* s, const void* W, void* Wn, const StepArgs& a)
+static hipError_t launch_one(lhysea_sim* s, const void* W, void* Wn, const StepArgs& a0)
 {
+    StepArgs a = a0;
+    a.step = s->step_host++;
     if (s->p.precision == LHYSEA_FP64) {
         if (s->mode == MODE_FUSED) return f64_step_fused(s->k, (const double*)W, (double*)Wn, s->b64.H, a, s->stream);
         return f64_step_fourpass(s->k, (const double*)W, (double*)Wn, s->b64.H, s->b64.acc, a, s->stream);
@@ -675,6 +679,7 @@ hipError_t hook_advance(void* c, int k0, int n)
 {
     lhysea_adj_work* a = (lhysea_adj_work*)c;
     hipError_t e = adj_set_step(a->s->ctl, k0, a->s->stream);
+    a->s->step_host = k0;
     // the restored state's tile classes (the launch split of the forward pass is kept)
     if (e == hipSuccess) e = classify(a->s, false);
     if (e != hipSuccess) return e;

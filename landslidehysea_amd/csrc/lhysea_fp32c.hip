@@ -757,8 +757,10 @@ __global__ void __launch_bounds__(NT) k_fused_gen(Consts k, const float* __restr
     constexpr int NSUB = (FTX + STX - 1) / STX;
     static_assert(TY == FTY && STX % 2 == 0, "sub-tiles of a class tile");
     Ctl* ctl = a.ctl;
-    if (!(ctl->t < a.t_end)) return;
-    const int s = ctl->step;
+    // the step is the launch's own: a workgroup that starts after the last
+    // participant closed it would otherwise read the next step's list
+    const int s = a.step;
+    if (!(ctl->t < a.t_end) || ctl->step != s) return;
     const int n = NSUB * (int)ctl->gcount[s & 1];
     // workgroups with work items (at least one, which then closes the step)
     const unsigned part = (unsigned)max(min(n, (int)gridDim.x), 1);

@@ -673,8 +673,10 @@ __global__ void __launch_bounds__(NT) k_fused_gen(Consts k, const double* __rest
                                                   const double* __restrict__ H, StepArgs a, TileCls tc)
 {
     Ctl* ctl = a.ctl;
-    if (!(ctl->t < a.t_end)) return;
-    const int s = ctl->step;
+    // the step is the launch's own: a workgroup that starts after the last
+    // participant closed it would otherwise read the next step's list
+    const int s = a.step;
+    if (!(ctl->t < a.t_end) || ctl->step != s) return;
     const int n = (int)ctl->gcount[s & 1];
     const unsigned part = (unsigned)max(min(n, (int)gridDim.x), 1);
     if (blockIdx.x >= part) return;

@@ -38,6 +38,7 @@ struct StepArgs {
     int* tcls;              // [2][ntiles]
     int* tlist;             // [2][ntiles]
     int split;              // 1: sediment-free and general launches, 0: every tile general
+    int step;               // the step this launch performs (ctl->step when it starts)
 };
 
 // Device buffers of one simulation in FP64.
