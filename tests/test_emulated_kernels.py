@@ -1,0 +1,178 @@
+"""The library's kernel sources run on the CPU under an emulation of HIP.
+
+tests/native/emu holds a host-side stand-in for <hip/hip_runtime.h> and its
+runtime: the .hip sources compile with g++ into a library with the same C ABI,
+whose kernels run workgroup by workgroup in index order, the lanes of a
+workgroup as cooperative fibers.  That order is the most adverse one for the
+protocols between the workgroups of one launch (the general-tile launch that
+closes a step, the class lists written by one step and read by the next): a
+workgroup starts only after every lower-numbered one has finished, the way a
+workgroup that the GPU schedules late would.  A general-tile launch whose late
+workgroups read the next step's tile list instead of leaving fails the FP32C
+check on the 64 x 48 grid (whose general set grows) and the FP64 check by
+metres of eta.
+
+Checks, bit for bit between paths of the library on the same inputs:
+  * FP32C split step (sediment-free and general launches) == every tile
+    through the general tile function == the batched kernel, on a ragged grid
+    of several class tiles with slides that cross tile edges;
+  * FP64 split step (forced on and off) == the four-pass kernels;
+  * run_until with the split forced on == the all-tiles launch, both precisions;
+and against the FP64 oracle (C restatement), the emulated FP64 step within
+1e-10 relative L2 over the state.
+
+The emulated library is built with -ffp-contract=off, so its arithmetic is not
+that of the gfx950 code object; the GPU tests repeat these checks on the
+device.  The cases run in a child process so the emulated library never shares
+a process with the HIP one.
+"""
+from __future__ import annotations
+
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(ROOT, "landslidehysea_amd", "csrc")
+SOURCES = ["lhysea_capi.hip", "lhysea_batch.hip", "lhysea_fp64.hip", "lhysea_fp32c.hip", "lhysea_adjoint.hip"]
+
+
+def build_emulated(out: str):
+    cmd = ["g++", "-std=c++20", "-O1", "-fPIC", "-shared", "-ffp-contract=off", "-w",
+           "-I", os.path.join(HERE, "native", "emu"), "-x", "c++"]
+    cmd += [os.path.join(CSRC, s) for s in SOURCES]
+    cmd += ["-x", "none", os.path.join(HERE, "native", "emu", "emu_rt.cpp"), "-o", out]
+    subprocess.run(cmd, check=True)
+
+
+def run_cases(lib: str) -> dict:
+    """Child process: every case on the emulated library; returns the verdicts."""
+    sys.path.insert(0, ROOT)
+    from landslidehysea_amd import solver as L
+    from landslidehysea_amd.scenario import gauge_cells
+    from tests.cases import case, oracle_params, rel_l2
+    from oracle import binding as ob
+
+    L.LIB_PATH = lib
+    L._lib = None
+    res = {}
+
+    def single(W, H, prm, steps, gauges, split, kernel=L.KERNEL_FUSED):
+        with L.Simulation(prm, kernel=kernel) as sim:
+            if split is not None:
+                sim.set_tile_split(split)
+            sim.upload(W, H)
+            sim.track_eta_max(True)
+            sim.set_gauges(gauges, steps)
+            g0 = sim.tile_classes()
+            sim.step(steps)
+            return {"W": sim.download(), "time": sim.time(), "dt": sim.dt_history(steps), "gauges": sim.gauges(),
+                    "emax": sim.eta_max(), "classes": (g0, sim.tile_classes())}
+
+    def same(a, b):
+        return bool(np.array_equal(a["W"], b["W"]) and a["time"] == b["time"] and np.array_equal(a["dt"], b["dt"])
+                    and np.array_equal(a["gauges"], b["gauges"]) and np.array_equal(a["emax"][0], b["emax"][0])
+                    and np.array_equal(a["emax"][1], b["emax"][1]))
+
+    # FP32C: 157 x 61 cells = 3 x 5 class tiles of 62 x 14, slides across tile
+    # edges; 64 x 48 cells = 2 x 4 class tiles, where the general set grows
+    for name, nx, ny, seed, angle, axes, steps in (("a", 157, 61, 5, 1.0, (12.0, 7.0), 12),
+                                                  ("b", 64, 48, 1, 2.0, (12.0, 9.0), 8)):
+        sc, W, H, prm = case(nx, ny, seed=seed, angle=angle, n_slides=3 if name == "a" else 1, semi_axes=axes,
+                             thickness=(30.0, 60.0) if name == "a" else (40.0, 40.0), precision=L.FP32C,
+                             borde=(1.0, -1.0, 1.0, -1.0) if name == "a" else (-1.0, -1.0, -1.0, -1.0))
+        gauges = gauge_cells(sc, 6, seed=seed)
+        with L.Batch([prm]) as b:
+            b.upload(0, W, H)
+            b.track_eta_max(0, True)
+            b.set_gauges(0, gauges, steps)
+            b.initial_dt()
+            b.step(steps)
+            ref = {"W": b.download(0), "time": b.time(0), "emax": b.eta_max(0), "dt": b.dt_history(0, steps),
+                   "gauges": b.gauges(0)}
+        for split in (1, 0):
+            a = single(W, H, prm, steps, gauges, split)
+            (nt, g0, _), (_, g1, sp) = a["classes"]
+            res[f"fp32c_{name}_split{split}"] = {"same": same(a, ref), "tiles": nt, "general": [g0, g1],
+                                                 "split": sp, "steps": a["time"][2], "want_steps": steps}
+
+    # FP64: split forced on and off against the four-pass kernels
+    sc, W, H, prm = case(131, 57, seed=6, angle=1.0, n_slides=3, semi_axes=(12.0, 7.0), thickness=(30.0, 60.0),
+                         borde=(1.0, -1.0, 1.0, -1.0))
+    gauges = gauge_cells(sc, 6, seed=6)
+    ref = single(W, H, prm, steps, gauges, None, kernel=L.KERNEL_FOURPASS)
+    for split in (1, 0):
+        a = single(W, H, prm, steps, gauges, split)
+        nt, g, sp = a["classes"][1]
+        res[f"fp64_split{split}"] = {"same": same(a, ref), "tiles": nt, "general": g, "split": sp}
+    Wo = W.copy()
+    ob.run(oracle_params(ob, prm), Wo, H, steps)
+    res["fp64_oracle"] = {"rel_l2": rel_l2(a["W"], Wo)}
+
+    # run_until with the split forced on and off
+    for prec in (L.FP32C, L.FP64):
+        sc, W, H, prm = case(131, 57, seed=9, angle=1.0, n_slides=2, semi_axes=(12.0, 7.0), thickness=(30.0, 60.0),
+                             precision=prec)
+        out = {}
+        for split in (1, 0):
+            with L.Simulation(prm) as sim:
+                sim.set_tile_split(split)
+                sim.upload(W, H)
+                n1 = sim.run_until(3.0)
+                n2 = sim.run_until(6.0, 7)
+                out[split] = (sim.download(), sim.time(), n1, n2)
+        a, b = out[1], out[0]
+        res[f"run_until_{prec}"] = {"same": bool(np.array_equal(a[0], b[0]) and a[1:] == b[1:]),
+                                    "steps": [a[2], a[3]]}
+    return res
+
+
+@pytest.fixture(scope="module")
+def emulated(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("needs g++")
+    lib = str(tmp_path_factory.mktemp("emu") / "liblhysea_emu.so")
+    build_emulated(lib)
+    out = subprocess.run([sys.executable, __file__, lib], capture_output=True, text=True, timeout=600, cwd=ROOT)
+    assert out.returncode == 0, out.stdout + out.stderr
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    print(json.dumps(res, indent=1))
+    return res
+
+
+def test_fp32c_split_step_in_workgroup_order(emulated):
+    for name, tiles in (("a", 15), ("b", 8)):
+        for split in (1, 0):
+            r = emulated[f"fp32c_{name}_split{split}"]
+            assert r["same"], r
+            assert r["tiles"] == tiles and r["split"] == bool(split) and r["steps"] == r["want_steps"]
+            assert all(0 < g < r["tiles"] for g in r["general"])
+    # the general set grows during the 64 x 48 run: the step after lists more tiles
+    assert emulated["fp32c_b_split1"]["general"][1] > emulated["fp32c_b_split1"]["general"][0]
+
+
+def test_fp64_split_step_in_workgroup_order(emulated):
+    for split in (1, 0):
+        r = emulated[f"fp64_split{split}"]
+        assert r["same"], r
+        assert 0 < r["general"] < r["tiles"] and r["split"] == bool(split)
+
+
+def test_emulated_fp64_step_matches_oracle(emulated):
+    assert emulated["fp64_oracle"]["rel_l2"] <= 1e-10
+
+
+def test_run_until_split_in_workgroup_order(emulated):
+    for prec in (1, 0):
+        r = emulated[f"run_until_{prec}"]
+        assert r["same"] and r["steps"][0] > 5 and r["steps"][1] == 7, r
+
+
+if __name__ == "__main__":
+    print(json.dumps(run_cases(sys.argv[1])))
