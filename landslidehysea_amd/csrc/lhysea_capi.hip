@@ -129,6 +129,7 @@ static StepArgs step_args(lhysea_sim* s)
     a.tcls = s->tcls;
     a.tlist = s->tlist;
     a.split = s->split;
+    a.wn_prev = 1;
     return a;
 }
 
@@ -798,7 +799,8 @@ int lhysea_adjoint_forward(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const i
     ADJCHK(hipEventRecord(a->e0, s->stream));
     ADJCHK(hook_store(a, 0));
     if (a->store_all) {
-        const StepArgs sa = launch_args(s, nullptr, nullptr);
+        StepArgs sa = launch_args(s, nullptr, nullptr);
+        sa.wn_prev = 0;    // each step writes a fresh slot
         for (long k = 0; k < nsteps; k++)
             ADJCHK(launch_one(s, a->snap + (size_t)k * a->state_bytes, a->snap + (size_t)(k + 1) * a->state_bytes, sa));
     } else {

@@ -38,6 +38,9 @@ struct Ctl {
     double t;                       // simulated time at the start of the running step
     double maxA_last;               // max A of the last completed step (diagnostic)
     unsigned int gcount[2];         // general tiles listed for the steps of parity 0 / 1 (FP32C)
+    // step at which the class tables were last rebuilt from a state (upload,
+    // restore); the tables describe every step from it on
+    int cls_from = 0x7fffffff;
 };
 
 // dt from the wave-speed max: min_i dt_i with dt_i = 2 CFL area / A_i, or 1e30

@@ -495,11 +495,15 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
 // halves the LDS footprint and the registers the general two-layer path needs.
 // A tile whose class entry *cls equals s belongs to the general kernel: the
 // block returns false before its first LDS store (its loads were issued).
+// From step skip_from + 2 on, Wn holds the state that step s - 2 wrote; a
+// tile that step took as sediment-free (its entry, stamped for s - 2 and not
+// raised to s, differs from s - 2) has zero layer-2 planes there already, so
+// only the three layer-1 planes and their compensations are written.
 template <int TX, int TY, int NT>
 __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __restrict__ W, float* __restrict__ Wn,
                                                const double* __restrict__ H, double dt, double tnow,
                                                double* __restrict__ emax, int i0, int j0, const int* cls, int s,
-                                               GaugeOut go, double& Amax)
+                                               int skip_from, GaugeOut go, double& Amax)
 {
     static_assert(TX % 2 == 0 && TY % 2 == 0, "tile origins must stay even");
     constexpr int CW = TX + 2, RW = TY + 2, CWP = CW + 1;
@@ -515,6 +519,7 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
 
     constexpr int CU_IT = (TX * TY + NT - 1) / NT, ST_IT = (RW * CW + NT - 1) / NT;
     float lo1x[CU_IT], lo1y[CU_IT];
+    bool zero2;
 #pragma unroll
     for (int it = 0; it < CU_IT; it++) {
         const int e = tid + it * NT;
@@ -544,7 +549,9 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
                 vq[it][1] = W[2 * N + g];
             }
         }
-        if (*cls == s) return false;
+        const int v = *cls;
+        if (v == s) return false;
+        zero2 = !(s - 2 >= skip_from && v != s - 2);
 #pragma unroll
         for (int it = 0; it < ST_IT; it++) {
             if (!in[it]) continue;
@@ -697,8 +704,13 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
         for (int f = 0; f < 3; f++) {
             Wn[f * N + g] = o[f].hi;
             Wn[(6 + f) * N + g] = o[f].lo;
-            Wn[(3 + f) * N + g] = 0.0f;
-            Wn[(9 + f) * N + g] = 0.0f;
+        }
+        if (zero2) {
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                Wn[(3 + f) * N + g] = 0.0f;
+                Wn[(9 + f) * N + g] = 0.0f;
+            }
         }
         // the new h1 replaces this cell's staged one (read by no other thread now)
         if (go.out) sh1[r + 1][c + 1] = (double)o[0].hi + (double)o[0].lo;
@@ -733,7 +745,7 @@ __global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const float* __rest
     const GaugeOut go{a.gauges, a.ng, (a.gseries && s < a.gcap) ? a.gseries + (size_t)s * a.ng : nullptr};
     double Amax;
     if (!tile_step_bare<TX, TY, NT>(k, W, Wn, H, dt, a.ctl->t, a.emax, tx * TX, ty * TY, tc.cls + (s & 1) * nb + b,
-                                    s, go, Amax))
+                                    s, a.wn_prev ? a.ctl->cls_from : 0x7fffffff, go, Amax))
         return;
     block_max_atomic<NT>(Amax, &a.ctl->maxA_next);
 }
@@ -870,6 +882,7 @@ __global__ void __launch_bounds__(256) k_classify(Consts k, const float* __restr
     }
     m = block_or(m);
     if (m && threadIdx.x < 64) mark_general(tc, a.ctl, tx, ty, a.ctl->step, m);
+    if (t == 0 && threadIdx.x == 0) a.ctl->cls_from = a.ctl->step;
 }
 
 // Forward recomputation for the discrete adjoint: the step's partial and
@@ -948,7 +961,8 @@ __global__ void __launch_bounds__(NT) k_batch_bare(const BatchScen* __restrict__
     double Amax;
     if (!tile_step_bare<TX, TY, NT>(k, gmem(d.W[cur]), gmem(d.W[cur ^ 1]), gmem(d.H), ctl->dt_cur, ctl->t,
                                     d.emax ? gmem(d.emax) : nullptr, tx * TX, ty * TY,
-                                    gmem(d.cls) + (step & 1) * nct + tile, step, GaugeOut{nullptr, 0, nullptr}, Amax))
+                                    gmem(d.cls) + (step & 1) * nct + tile, step, ctl->cls_from,
+                                    GaugeOut{nullptr, 0, nullptr}, Amax))
         return;
     block_max_atomic<NT>(Amax, &ctl->maxA_next);
 }
@@ -1054,6 +1068,7 @@ __global__ void __launch_bounds__(256) k_batch_classify(const BatchScen* __restr
     }
     m = block_or(m);
     if (m && threadIdx.x < 64) mark_general(TileCls{d.cls, d.list, d.ctiles_x, d.ctiles_y}, d.ctl, tx, ty, step, m);
+    if (t == 0 && threadIdx.x == 0) d.ctl->cls_from = step;
 }
 
 // One block per scenario: gauges of the new state, dt history, t += dt, the

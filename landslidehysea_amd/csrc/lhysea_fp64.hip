@@ -461,11 +461,14 @@ constexpr int GGRID = 1024;
 // the results are the same; the layer-2 planes are neither read nor kept in
 // LDS (zeros are written).  A tile whose class entry *cls equals s belongs to
 // the general kernel: the block returns false before its first LDS store.
+// From step skip_from + 2 on, Wn holds the state that step s - 2 wrote; a
+// tile that step took as sediment-free (entry other than s - 2) has zero
+// layer-2 planes there already, and only the layer-1 planes are written.
 template <int TX, int TY, int NT>
 __device__ __forceinline__ bool tile_step_bare64(const Consts& k, const double* __restrict__ W,
                                                  double* __restrict__ Wn, const double* __restrict__ H, double dt,
                                                  double tnow, double* __restrict__ emax, int i0, int j0,
-                                                 const int* cls, int s, GaugeOut go, double& Amax)
+                                                 const int* cls, int s, int skip_from, GaugeOut go, double& Amax)
 {
     constexpr int CW = TX + 2, RW = TY + 2;
     __shared__ double sW[4][RW][CW];   // h1, q1x, q1y, H
@@ -491,7 +494,9 @@ __device__ __forceinline__ bool tile_step_bare64(const Consts& k, const double* 
             v[it][3] = H[g];
         }
     }
-    if (*cls == s) return false;
+    const int cv = *cls;
+    if (cv == s) return false;
+    const bool zero2 = !(s - 2 >= skip_from && cv != s - 2);
 #pragma unroll
     for (int it = 0; it < ST_IT; it++) {
         if (!in[it]) continue;
@@ -613,7 +618,11 @@ __device__ __forceinline__ bool tile_step_bare64(const Consts& k, const double* 
         }
         cell_update1(k, o, ac, dt, n);
 #pragma unroll
-        for (int f = 0; f < 6; f++) Wn[f * N + g] = n[f];
+        for (int f = 0; f < 3; f++) Wn[f * N + g] = n[f];
+        if (zero2) {
+#pragma unroll
+            for (int f = 3; f < 6; f++) Wn[f * N + g] = n[f];
+        }
         if (go.out) sW[0][r + 1][c + 1] = n[0];   // the new h1 (read by no other thread now)
     }
     if (go.out) gauges_of_tile<TX, TY, NT>(go, &sW[0][0][0], nullptr, &sW[3][0][0], CW, nx, i0, j0, 0x7fffffff);
@@ -659,7 +668,8 @@ __global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const double* __res
     const GaugeOut go{a.gauges, a.ng, (a.gseries && s < a.gcap) ? a.gseries + (size_t)s * a.ng : nullptr};
     double Amax;
     if (!tile_step_bare64<TX, TY, NT>(k, W, Wn, H, step_dt(a.ctl, a.dt_seq), a.ctl->t, a.emax, tx * TX, ty * TY,
-                                      tc.cls + (s & 1) * nb + b, s, go, Amax))
+                                      tc.cls + (s & 1) * nb + b, s, a.wn_prev ? a.ctl->cls_from : 0x7fffffff, go,
+                                      Amax))
         return;
     block_max_atomic<NT>(Amax, &a.ctl->maxA_next);
 }
@@ -751,6 +761,7 @@ __global__ void __launch_bounds__(256) k_classify(Consts k, const double* __rest
     }
     m = block_or(m);
     if (m && threadIdx.x < 64) mark_general(tc, a.ctl, tx, ty, a.ctl->step, m);
+    if (t == 0 && threadIdx.x == 0) a.ctl->cls_from = a.ctl->step;
 }
 
 }  // namespace f64

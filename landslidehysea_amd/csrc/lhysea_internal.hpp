@@ -39,6 +39,11 @@ struct StepArgs {
     int* tlist;             // [2][ntiles]
     int split;              // 1: sediment-free and general launches, 0: every tile general
     int step;               // the step this launch performs (ctl->step when it starts)
+    // 1: Wn holds the state two steps back (ping-pong buffers), whose layer-2
+    // planes are zero on every tile that step s - 2 took as sediment-free, so
+    // the sediment-free launch need not write them again; 0: Wn is a fresh
+    // buffer (every state kept), all planes are written
+    int wn_prev;
 };
 
 // Device buffers of one simulation in FP64.

@@ -18,6 +18,9 @@ Checks, bit for bit between paths of the library on the same inputs:
     of several class tiles with slides that cross tile edges;
   * FP64 split step (forced on and off) == the four-pass kernels;
   * run_until with the split forced on == the all-tiles launch, both precisions;
+  * a second upload into a handle whose buffers hold sediment everywhere ==
+    a fresh handle, both precisions (the sediment-free launch rewrites the
+    layer-2 planes until its own writes are two steps back);
 and against the FP64 oracle (C restatement), the emulated FP64 step within
 1e-10 relative L2 over the state.
 
@@ -102,6 +105,26 @@ def run_cases(lib: str) -> dict:
             res[f"fp32c_{name}_split{split}"] = {"same": same(a, ref), "tiles": nt, "general": [g0, g1],
                                                  "split": sp, "steps": a["time"][2], "want_steps": steps}
 
+    # a second upload into a handle whose buffers hold sediment everywhere:
+    # the first steps after it must rewrite the layer-2 planes of the
+    # sediment-free tiles (the other buffer holds the old run's state)
+    for prec in (L.FP32C, L.FP64):
+        sc, W, H, prm = case(157, 61, seed=5, angle=1.0, n_slides=3, semi_axes=(12.0, 7.0),
+                             thickness=(30.0, 60.0), precision=prec)
+        Wsed = W.copy()
+        Wsed[3] = np.maximum(Wsed[3], 0.01)
+        outs = []
+        for reuse in (True, False):
+            with L.Simulation(prm) as sim:
+                sim.set_tile_split(1)
+                if reuse:
+                    sim.upload(Wsed, H)
+                    sim.step(3)
+                sim.upload(W, H)
+                sim.step(6)
+                outs.append((sim.download(), sim.time()))
+        res[f"reupload_{prec}"] = {"same": bool(np.array_equal(outs[0][0], outs[1][0]) and outs[0][1] == outs[1][1])}
+
     # FP64: split forced on and off against the four-pass kernels
     sc, W, H, prm = case(131, 57, seed=6, angle=1.0, n_slides=3, semi_axes=(12.0, 7.0), thickness=(30.0, 60.0),
                          borde=(1.0, -1.0, 1.0, -1.0))
@@ -162,6 +185,11 @@ def test_fp64_split_step_in_workgroup_order(emulated):
         r = emulated[f"fp64_split{split}"]
         assert r["same"], r
         assert 0 < r["general"] < r["tiles"] and r["split"] == bool(split)
+
+
+def test_reupload_rewrites_layer2_in_workgroup_order(emulated):
+    for prec in (1, 0):
+        assert emulated[f"reupload_{prec}"]["same"]
 
 
 def test_emulated_fp64_step_matches_oracle(emulated):

@@ -141,3 +141,28 @@ def test_split_step_run_until(precision):
     say("run_until steps", a[2], a[3], "time", a[1])
     assert a[2] == b[2] and a[3] == b[3] and a[1] == b[1]
     assert np.array_equal(a[0], b[0])
+
+
+@pytest.mark.parametrize("precision", [L.FP32C, L.FP64])
+def test_reupload_over_sediment_equals_fresh_handle(precision):
+    """The sediment-free launch leaves the layer-2 planes of a tile unwritten
+    once its own zeros are in the output buffer (the tile was sediment-free two
+    steps back, after the last classification).  A second upload into a handle
+    whose buffers hold sediment everywhere must still match a fresh handle bit
+    for bit, and so must a long run where slides leave tiles behind."""
+    sc, W, H, prm = case(400, 300, seed=5, angle=1.0, n_slides=3, semi_axes=(14.0, 9.0), thickness=(30.0, 60.0),
+                         precision=precision, borde=(1.0, -1.0, 1.0, -1.0))
+    Wsed = W.copy()
+    Wsed[3] = np.maximum(Wsed[3], 0.01)
+    outs = []
+    for reuse in (True, False):
+        with L.Simulation(prm) as sim:
+            sim.set_tile_split(1)
+            if reuse:
+                sim.upload(Wsed, H)
+                sim.step(5)
+            sim.upload(W, H)
+            sim.step(120)
+            outs.append((sim.download(), sim.time()))
+    say(f"reupload precision={precision}: time {outs[0][1]}")
+    assert np.array_equal(outs[0][0], outs[1][0]) and outs[0][1] == outs[1][1]
