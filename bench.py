@@ -40,8 +40,9 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE (x1.0), the scale factors of a plain 256 MiB copy
 # (profiles/fused_step_fp32c_2048x2048_counters.txt,
 # profiles/copy_256MiB_counter_calibration.txt); below the algorithmic bytes
-# because sediment-free tiles do not read their zero layer-2 planes
-TRAFFIC_BYTES_FP32C = 353.4e6
+# because sediment-free tiles neither read their zero layer-2 planes nor write
+# them again where the output buffer already holds them
+TRAFFIC_BYTES_FP32C = 256.2e6
 NX = NY = 2048
 SEED = 2048
 
