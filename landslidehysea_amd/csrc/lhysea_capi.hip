@@ -267,7 +267,11 @@ int lhysea_create(const lhysea_params* p, lhysea_sim** out)
         return cleanup(LHYSEA_ERR_GPU_MEMORY, "device allocation failed");
     if (hipMalloc(&s->ctl, sizeof(Ctl)) != hipSuccess || hipMalloc(&s->emax, 2 * N * sizeof(double)) != hipSuccess)
         return cleanup(LHYSEA_ERR_GPU_MEMORY, "device allocation failed");
-    hipMemsetAsync(s->ctl, 0, sizeof(Ctl), s->stream);
+    {
+        const Ctl c0{};   // no class table yet (cls_from past every step)
+        if (hipMemcpy(s->ctl, &c0, sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(LHYSEA_ERR_HIP, "control block upload failed");
+    }
     s->hcap = 1 << 20;
     if (hipMalloc(&s->dt_hist, (size_t)s->hcap * sizeof(double)) != hipSuccess)
         return cleanup(LHYSEA_ERR_GPU_MEMORY, "device allocation failed");
