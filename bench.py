@@ -14,6 +14,10 @@ HBM; the FP64 path is timed beside it.  roofline.achieved uses algorithmic
 bytes per cell-update (104 B: read 6 state fields + depth, write 6 fields)
 over the step kernel's average duration measured with HIP events on the
 library's stream.  cpu_baseline times the FP64 oracle on the host cores.
+precision_2048 reports FP32C against FP64 on the same construction (gauges,
+eta, h_s, maximum runup) at the workload's 10 deg and with the slides flowing
+at 2 deg, beside the spread that FP32-sized rounding of the input alone causes
+in the FP64 solver.
 """
 from __future__ import annotations
 
@@ -22,6 +26,7 @@ import json
 import os
 import sys
 import time
+from dataclasses import replace
 
 import numpy as np
 
@@ -92,6 +97,52 @@ def time_adjoint(precision, steps=4000, snaps=64, n=1024, ngauges=16):
             "reverse_ms": round(rev, 2), "cost_vs_forward": round((fwd + rev) / fwd, 3),
             "mcells_fwd_plus_adjoint": round(n * n * steps / ((fwd + rev) * 1e-3) / 1e6, 2),
             "re_advances": res["advances"], "wall_s": round(wall, 2)}
+
+
+def precision_report(steps=1000, angle=2.0, ngauges=16):
+    """FP32C against FP64 on the 2048x2048 construction: relative L2 of the
+    gauge signal (eta - sea level at rest; gauges among the wet cells within
+    40 cells of a slide), of the eta field and of h_s, and the maximum runup;
+    beside it the spread that rounding the initial h_s to FP32 size
+    (x (1 + 1e-7 N(0,1))) alone causes in the FP64 solver."""
+    from scipy import ndimage
+    sc, W, H, p64 = make_case()
+    p64 = replace(p64, angle_deg=(angle,))
+    p32 = replace(p64, precision=L.FP32C)
+    near = ndimage.distance_transform_edt(W[3] <= 0.0) <= 40.0
+    cand = np.flatnonzero((near & (W[3] <= 0.0) & (W[0] > 1.0)).ravel())
+    gauges = np.sort(np.random.default_rng(16).choice(cand, ngauges, replace=False)).astype(np.int32)
+
+    def run(W0, prm):
+        with L.Simulation(prm) as sim:
+            sim.upload(W0, H)
+            sim.track_eta_max(True)
+            sim.set_gauges(gauges, steps)
+            sim.step(steps)
+            return sim.download(), sim.gauges(), sim.eta_max()[0]
+
+    def rel(a, b):
+        d = np.linalg.norm(b)
+        return float(np.linalg.norm(a - b) / d) if d > 0 else float(np.linalg.norm(a - b))
+
+    W64, g64, e64 = run(W, p64)
+    W32, g32, e32 = run(W, p32)
+    Wp = W.copy()
+    Wp[3] = Wp[3] * (1.0 + 1e-7 * np.random.default_rng(7).standard_normal(Wp[3].shape))
+    Wq, gq, _ = run(Wp, p64)
+    rest = float((W[0] + W[3] - H)[W[0] > 1.0].mean())
+    wet = W64[0] + W64[3] > 0.0
+
+    def errs(Wx, gx):
+        return {"gauges": rel(gx - rest, g64 - rest),
+                "eta": rel(Wx[0] + Wx[3] - H - rest, W64[0] + W64[3] - H - rest),
+                "h_s": rel(Wx[3], W64[3])}
+
+    return {"steps": steps, "friction_deg": angle, "gauges": ngauges,
+            "gauge_signal_max_m": float(np.abs(g64 - rest).max()),
+            "fp32c_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs(W32, g32).items()},
+            "fp64_input_rounding_spread": {k: float(f"{v:.3e}") for k, v in errs(Wq, gq).items()},
+            "max_runup_m": {"fp64": float(e64[wet].max()) - rest, "fp32c": float(e32[wet].max()) - rest}}
 
 
 def time_large(steps=100, adj_steps=200, snaps=100, n=4096):
@@ -240,6 +291,8 @@ def main():
                     help="timed steps of the 64-scenario ensemble workload (0 skips it)")
     ap.add_argument("--large-steps", type=int, default=100,
                     help="timed forward steps of the 4096x4096 workload (0 skips it)")
+    ap.add_argument("--precision-steps", type=int, default=1000,
+                    help="steps of the FP32C-vs-FP64 precision report at 2048x2048 (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the whole grid lives in one MI355X's HBM); "
@@ -298,6 +351,8 @@ def main():
         out["ensemble"] = time_ensemble(args.ensemble_steps)
     if args.large_steps > 0 and fp32c:
         out["large_4096"] = time_large(args.large_steps)
+    if args.precision_steps > 0 and fp32c and fp64:
+        out["precision_2048"] = [precision_report(args.precision_steps, a) for a in (10.0, 2.0)]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
