@@ -150,3 +150,89 @@ def test_precision_comparison_2048(angle):
     for k in ("gauge", "eta", "h_s"):
         assert e32[k] <= max(1e-4, 2.0 * ep[k]), (k, e32[k], ep[k])
     assert abs(run32 - run64) <= max(1e-4, 2.0 * ep["eta"]) * abs(run64)
+
+
+@pytest.mark.parametrize("angle", [10.0, 2.0])
+def test_adjoint_1024_matches_central_fd(angle):
+    """Discrete adjoint at the adjoint workload's size (1024 x 1024, one slide,
+    at 10 deg and flowing at 2 deg; 300 steps, 16 HBM snapshots, 16 gauges
+    near the slide, misfit against a forward run from a 2% thicker slide)
+    against central finite differences of FP64 device forward runs on 32
+    seeded sediment cells (steps 1e-4 and 5e-5 of the cell's thickness) and on
+    the friction angle.  The adjoint freezes dt; every perturbed run is
+    checked to keep the base run's dt sequence.
+
+    At this size a step of 1e-4 of the thickness moves Coulomb stop/go and
+    wet/dry decisions of other cells somewhere downstream, so the loss is
+    smooth only piecewise on the scale of the step and the differences hold
+    only to ~1e-3 where the two steps agree (the small-grid test in
+    test_gpu_adjoint.py pins the adjoint's arithmetic to 1e-4).  Bars: cells
+    whose two differences agree to 1e-3 match the finer one to 5e-3, with
+    median <= 1e-3; the angle derivative matches to 1e-2; the other cells are
+    reported."""
+    n, steps = 1024, 300
+    sc = S.make_scenario(n, n, seed=1024, n_slides=1)
+    H, _ = S.hmin_shift(sc.H)
+    H = np.ascontiguousarray(H)
+    W = np.ascontiguousarray(sc.state)
+    prm = L.Params(nx=n, ny=n, dx=sc.dx, dy=sc.dy, angle_deg=(angle,), precision=L.FP64)
+    near = ndimage.distance_transform_edt(W[3] <= 0.0) <= 30.0
+    cand = np.flatnonzero((near & (W[3] <= 0.0) & (W[0] > 1.0)).ravel())
+    gauges = np.sort(np.random.default_rng(16).choice(cand, 16, replace=False)).astype(np.int32)
+
+    def forward(W0, p):
+        with L.Simulation(p) as sim:
+            sim.upload(W0, H)
+            sim.set_gauges(gauges, steps)
+            sim.step(steps)
+            return sim.gauges(), sim.dt_history(steps)
+
+    Wt = W.copy()                                   # the observed run: a 2% thicker slide at rest
+    surf = W[0] + W[3]
+    Wt[3] = 1.02 * W[3]
+    Wt[0] = np.where(W[0] > 0.0, np.maximum(surf - Wt[3], 0.0), 0.0)
+    obs, _ = forward(Wt, prm)
+    g0, dts0 = forward(W, prm)
+    L0 = float(((g0 - obs) ** 2).sum())
+    with L.Simulation(prm) as sim:
+        sim.upload(W, H)
+        res = sim.adjoint(steps, 16, gauges, obs)
+    say(f"adjoint 1024x1024 {angle} deg {steps} steps: loss {res['loss']:.9e} (forward runs {L0:.9e}), "
+        f"re-advances {res['advances']}")
+    assert abs(res["loss"] - L0) <= 1e-12 * L0
+
+    def loss_of(W0, p=prm):
+        g, dts = forward(W0, p)
+        return float(((g - obs) ** 2).sum()), np.array_equal(dts, dts0)
+
+    rng = np.random.default_rng(5)
+    cells = rng.choice(np.flatnonzero(W[3].ravel() > 5.0), 32, replace=False)
+    rels, other = [], 0
+    for c in cells:
+        j, i = divmod(int(c), n)
+        fcs = []
+        for eps in (1e-4 * W[3, j, i], 0.5e-4 * W[3, j, i]):
+            Wp, Wm = W.copy(), W.copy()
+            Wp[3, j, i] += eps
+            Wm[3, j, i] -= eps
+            (lp, okp), (lm, okm) = loss_of(Wp), loss_of(Wm)
+            assert okp and okm, f"cell {c}: the perturbation changes dt"
+            fcs.append((lp - lm) / (2 * eps))
+        ad = res["grad_W0"][3, j, i]
+        rel = abs(ad - fcs[1]) / abs(fcs[1])
+        if abs(fcs[0] - fcs[1]) <= 1e-3 * abs(fcs[1]):
+            rels.append(rel)
+        else:
+            other += 1
+            say(f"cell {c}: differences {fcs[0]:.6e} (eps) {fcs[1]:.6e} (eps/2), adjoint {ad:.6e}")
+    say(f"{len(rels)} cells with agreeing differences: adjoint vs FD max {max(rels):.2e} median "
+        f"{float(np.median(rels)):.2e}; {other} cells where a branch moves within the step")
+    assert len(rels) >= 4
+    assert max(rels) <= 5e-3 and float(np.median(rels)) <= 1e-3
+    da = 1e-7                                        # radians
+    lp, okp = loss_of(W, replace(prm, angle_deg=(angle + np.degrees(da),)))
+    lm, okm = loss_of(W, replace(prm, angle_deg=(angle - np.degrees(da),)))
+    fd = (lp - lm) / (2 * da)
+    say(f"angle: adjoint {res['grad_angle1']:.6e} central fd {fd:.6e} (dt kept: {okp and okm})")
+    assert okp and okm
+    assert abs(res["grad_angle1"] - fd) <= 1e-2 * abs(fd)
