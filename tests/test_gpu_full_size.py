@@ -1,4 +1,5 @@
-"""GPU checks at the full workload sizes (2048 x 2048 and 4096 x 4096).
+"""GPU checks at the full workload sizes (2048 x 2048, 4096 x 4096, the 1024 x 1024
+adjoint workload and the 64-scenario ensemble).
 
 The oracle needs ~0.2 s per 2048 x 2048 step on 16 host threads, so at these
 sizes it runs a few steps only; longer runs are checked through properties
@@ -236,3 +237,39 @@ def test_adjoint_1024_matches_central_fd(angle):
     say(f"angle: adjoint {res['grad_angle1']:.6e} central fd {fd:.6e} (dt kept: {okp and okm})")
     assert okp and okm
     assert abs(res["grad_angle1"] - fd) <= 1e-2 * abs(fd)
+
+
+def test_ensemble_64_equals_single_runs():
+    """The 64-scenario ensemble workload (sides 256..1536, 47 M cells) in one
+    batch, split shape and graph replays, 40 steps: the smallest, the largest
+    and two more scenarios equal their single-grid runs bit for bit (state,
+    time, dt history)."""
+    n, steps = 64, 40
+    specs, build = S.make_ensemble(n)
+    params = [L.Params(nx=sp["nx"], ny=sp["ny"], dx=S.DEFAULT_SPACING_DEG * S.DEG_TO_M,
+                       dy=S.DEFAULT_SPACING_DEG * S.DEG_TO_M, angle_deg=(sp["angle_deg"],), r=sp["r"],
+                       precision=L.FP32C) for sp in specs]
+    sizes = [p.nx * p.ny for p in params]
+    picks = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), 17, 42})
+    kept = {}
+    with L.Batch(params) as b:
+        for i, sp in enumerate(specs):
+            sc, _ = build(sp)
+            H, _ = S.hmin_shift(sc.H)
+            W, H = np.ascontiguousarray(sc.state), np.ascontiguousarray(H)
+            b.upload(i, W, H)
+            if i in picks:
+                kept[i] = (W, H)
+            del sc
+        b.initial_dt()
+        b.step(steps)
+        got = {i: (b.download(i), b.time(i), b.dt_history(i, steps)) for i in picks}
+    say(f"ensemble: {n} scenarios, {sum(sizes)} cells; compared {picks}")
+    for i in picks:
+        W, H = kept[i]
+        with L.Simulation(params[i]) as sim:
+            sim.upload(W, H)
+            sim.step(steps)
+            ref = (sim.download(), sim.time(), sim.dt_history(steps))
+        say(f"  scenario {i} {params[i].nx}x{params[i].ny}: time {ref[1]}")
+        assert np.array_equal(got[i][0], ref[0]) and got[i][1] == ref[1] and np.array_equal(got[i][2], ref[2])
