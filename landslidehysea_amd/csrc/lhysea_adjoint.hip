@@ -708,7 +708,8 @@ hipError_t reverse_step_t(const Consts& k, const F* W, const double* H, F* P, F*
         hipError_t e = f32c_recompute(k, (const float*)W, H, dt_hist, step, (float*)P, (float*)ACC, s);
         if (e != hipSuccess) return e;
     } else {
-        hipMemsetAsync(ACC, 0, 6 * N * sizeof(F), s);
+        hipError_t e = hipMemsetAsync(ACC, 0, 6 * N * sizeof(F), s);
+        if (e != hipSuccess) return e;
         for (int cls = 0; cls < 4; cls++)
             hipLaunchKernelGGL(adj::k_recompute<F>, dim3((unsigned)((ne[cls] + 255) / 256)), dim3(256), 0, s, k, sv,
                                H, w, dt_hist, step, cls);

@@ -44,7 +44,6 @@ struct lhysea_sim {
     lhysea_params p{};
     Consts k{};
     size_t N = 0;
-    int device = 0;
     hipStream_t stream = nullptr;
     int cur = 0;                 // buffer index holding the current state
     int mode = MODE_FUSED;
@@ -236,7 +235,6 @@ int lhysea_create(const lhysea_params* p, lhysea_sim** out)
     s->p = *p;
     s->k = make_consts(*p);
     s->N = (size_t)p->nx * p->ny;
-    hipGetDevice(&s->device);
     auto cleanup = [&](int code, const std::string& m) {
         lhysea_destroy(s);
         return fail(code, m);
@@ -251,7 +249,8 @@ int lhysea_create(const lhysea_params* p, lhysea_sim** out)
         if (hipMalloc(&s->b64.H, N * sizeof(double)) != hipSuccess ||
             hipMalloc(&s->b64.acc, 7 * N * sizeof(double)) != hipSuccess)
             return cleanup(LHYSEA_ERR_GPU_MEMORY, "device allocation failed");
-        hipMemsetAsync(s->b64.acc, 0, 7 * N * sizeof(double), s->stream);
+        if (hipMemsetAsync(s->b64.acc, 0, 7 * N * sizeof(double), s->stream) != hipSuccess)
+            return cleanup(LHYSEA_ERR_HIP, "accumulator clear failed");
         s->ntiles = f64_tiles(s->k);
     } else {
         for (int b = 0; b < 2; b++)
@@ -824,7 +823,7 @@ int lhysea_adjoint_forward(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const i
     ADJCHK(hipStreamSynchronize(s->stream));
     ADJCHK(hipGetLastError());
     float f_ms = 0;
-    hipEventElapsedTime(&f_ms, a->e0, a->e1);
+    (void)hipEventElapsedTime(&f_ms, a->e0, a->e1);   // a timing statistic: 0 if unavailable
     a->fwd_ms = f_ms;
     if (eta && ng > 0) {
         int32_t nrec = 0;
@@ -893,7 +892,7 @@ int lhysea_adjoint_backward(lhysea_sim* s, const double* seeds, double* grad_W0,
     }
     if (stats) {
         float r_ms = 0;
-        hipEventElapsedTime(&r_ms, a->e1, a->e2);
+        (void)hipEventElapsedTime(&r_ms, a->e1, a->e2);   // a timing statistic: 0 if unavailable
         stats[0] = a->fwd_ms;                  // forward pass (with first-descent snapshots)
         stats[1] = r_ms;                       // reverse sweep (re-advances + reverse steps)
         stats[2] = (double)h.advances;         // re-advanced forward steps

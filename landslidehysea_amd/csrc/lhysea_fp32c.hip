@@ -1140,17 +1140,18 @@ hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const dou
     const int tiles_y = (k.ny + f32::FTY - 1) / f32::FTY;
     const int nt = tiles_x * tiles_y;
     const TileCls tc{a.tcls, a.tlist, tiles_x, tiles_y};
-    if (a.ev0) hipEventRecord(a.ev0, s);
+    hipError_t e = a.ev0 ? hipEventRecord(a.ev0, s) : hipSuccess;
+    if (e != hipSuccess) return e;
     if (a.split) {
         hipLaunchKernelGGL((f32::k_fused_bare<f32::FTX, f32::FTY, f32::FNT>), dim3((unsigned)nt), dim3(f32::FNT), 0,
                            s, k, W, Wn, H, a, tc);
         hipLaunchKernelGGL((f32::k_fused_gen<f32::GSTX, f32::FTY, f32::GNT>), dim3(f32::GGRID), dim3(f32::GNT), 0, s,
                            k, W, Wn, H, a, tc);
-        if (a.ev1) hipEventRecord(a.ev1, s);
+        if (a.ev1 && (e = hipEventRecord(a.ev1, s)) != hipSuccess) return e;
     } else {
         hipLaunchKernelGGL((f32::k_fused_all<f32::FTX, f32::FTY, f32::FNT>), dim3((unsigned)nt), dim3(f32::FNT), 0, s,
                            k, W, Wn, H, a, tc);
-        if (a.ev1) hipEventRecord(a.ev1, s);
+        if (a.ev1 && (e = hipEventRecord(a.ev1, s)) != hipSuccess) return e;
         hipLaunchKernelGGL(f32::k_close, dim3(1), dim3(64), 0, s, k, a);
     }
     return hipGetLastError();

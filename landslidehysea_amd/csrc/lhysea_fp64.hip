@@ -816,17 +816,18 @@ hipError_t f64_step_fused(const Consts& k, const double* W, double* Wn, const do
     const int tiles_y = (k.ny + f64::FTY - 1) / f64::FTY;
     const int nt = tiles_x * tiles_y;
     const TileCls tc{a.tcls, a.tlist, tiles_x, tiles_y};
-    if (a.ev0) hipEventRecord(a.ev0, s);
+    hipError_t e = a.ev0 ? hipEventRecord(a.ev0, s) : hipSuccess;
+    if (e != hipSuccess) return e;
     if (a.split) {
         hipLaunchKernelGGL((f64::k_fused_bare<f64::FTX, f64::FTY, f64::FNT>), dim3((unsigned)nt), dim3(f64::FNT), 0,
                            s, k, W, Wn, H, a, tc);
         hipLaunchKernelGGL((f64::k_fused_gen<f64::FTX, f64::FTY, f64::FNT>), dim3(f64::GGRID), dim3(f64::FNT), 0, s,
                            k, W, Wn, H, a, tc);
-        if (a.ev1) hipEventRecord(a.ev1, s);
+        if (a.ev1 && (e = hipEventRecord(a.ev1, s)) != hipSuccess) return e;
     } else {
         hipLaunchKernelGGL((f64::k_fused_all<f64::FTX, f64::FTY, f64::FNT>), dim3((unsigned)nt), dim3(f64::FNT), 0,
                            s, k, W, Wn, H, a, tc);
-        if (a.ev1) hipEventRecord(a.ev1, s);
+        if (a.ev1 && (e = hipEventRecord(a.ev1, s)) != hipSuccess) return e;
         hipLaunchKernelGGL(f64::k_close, dim3(1), dim3(64), 0, s, k, a);
     }
     return hipGetLastError();
