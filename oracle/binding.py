@@ -1,7 +1,7 @@
 """ctypes binding of the FP64 oracle (oracle/liblhysea_oracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() (as
-the checker) and by bench.py's cpu_baseline leg.  The product package never
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() (to
+check the library's results) and by bench.py's cpu_baseline leg.  The product package never
 imports this module.
 """
 from __future__ import annotations

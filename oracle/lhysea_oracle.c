@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle: it is built
  * into oracle/liblhysea_oracle.so and may be loaded only by tests/, by
- * __graft_entry__.smoke() (as the checker) and by bench.py's cpu_baseline leg.
+ * __graft_entry__.smoke() (to check the library's results) and by bench.py's
+ * cpu_baseline leg.
  * The product path (landslidehysea_amd) never links, loads or calls it.
  *
  * It restates, in double precision and in the reference's own evaluation
