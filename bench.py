@@ -291,7 +291,7 @@ def main():
                     help="timed steps of the 64-scenario ensemble workload (0 skips it)")
     ap.add_argument("--large-steps", type=int, default=100,
                     help="timed forward steps of the 4096x4096 workload (0 skips it)")
-    ap.add_argument("--precision-steps", type=int, default=1000,
+    ap.add_argument("--precision-steps", type=int, default=10000,
                     help="steps of the FP32C-vs-FP64 precision report at 2048x2048 (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
