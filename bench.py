@@ -291,6 +291,8 @@ def main():
                     help="timed steps of the 64-scenario ensemble workload (0 skips it)")
     ap.add_argument("--large-steps", type=int, default=100,
                     help="timed forward steps of the 4096x4096 workload (0 skips it)")
+    ap.add_argument("--large-adjoint-steps", type=int, default=200,
+                    help="steps of the 4096x4096 adjoint (100 HBM snapshots; the workload's full length is 20000)")
     ap.add_argument("--precision-steps", type=int, default=10000,
                     help="steps of the FP32C-vs-FP64 precision report at 2048x2048 (0 skips it)")
     args = ap.parse_args()
@@ -350,7 +352,7 @@ def main():
     if args.ensemble_steps > 0 and fp32c:
         out["ensemble"] = time_ensemble(args.ensemble_steps)
     if args.large_steps > 0 and fp32c:
-        out["large_4096"] = time_large(args.large_steps)
+        out["large_4096"] = time_large(args.large_steps, args.large_adjoint_steps)
     if args.precision_steps > 0 and fp32c and fp64:
         out["precision_2048"] = [precision_report(args.precision_steps, a) for a in (10.0, 2.0)]
     if not args.no_cpu:
