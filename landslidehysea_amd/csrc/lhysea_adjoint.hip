@@ -55,7 +55,8 @@ struct Work {
 struct EdgeRef {
     size_t c0, c1;
     bool interior, xedge;
-    float n1x, n1y, len, bd;
+    float n1x, n1y;   // 0 or +-1
+    double len, bd;   // taken in the adjoint's precision
 };
 
 __device__ __forceinline__ bool edge_ref(const Consts& k, long e, int cls, EdgeRef& r)
@@ -66,14 +67,14 @@ __device__ __forceinline__ bool edge_ref(const Consts& k, long e, int cls, EdgeR
         if (e >= (long)nrow * nx) return false;
         const int je = cls + 2 * (int)(e / nx), i = (int)(e % nx);
         r.xedge = false;
-        r.len = (float)k.dx;
+        r.len = k.dx;
         r.n1x = 0.0f;
         if (je == 0 || je == ny) {
             r.c0 = (je == 0) ? (size_t)i : (size_t)(ny - 1) * nx + i;
             r.c1 = r.c0;
             r.interior = false;
             r.n1y = (je == 0) ? -1.0f : 1.0f;
-            r.bd = (float)((je == 0) ? k.borde_sup : k.borde_inf);
+            r.bd = (je == 0) ? k.borde_sup : k.borde_inf;
         } else {
             r.c0 = (size_t)(je - 1) * nx + i;
             r.c1 = r.c0 + nx;
@@ -86,14 +87,14 @@ __device__ __forceinline__ bool edge_ref(const Consts& k, long e, int cls, EdgeR
         if (e >= (long)ncol * ny) return false;
         const int j = (int)(e / ncol), ie = par + 2 * (int)(e % ncol);
         r.xedge = true;
-        r.len = (float)k.dy;
+        r.len = k.dy;
         r.n1y = 0.0f;
         if (ie == 0 || ie == nx) {
             r.c0 = (ie == 0) ? (size_t)j * nx : (size_t)j * nx + nx - 1;
             r.c1 = r.c0;
             r.interior = false;
             r.n1x = (ie == 0) ? -1.0f : 1.0f;
-            r.bd = (float)((ie == 0) ? k.borde_izq : k.borde_der);
+            r.bd = (ie == 0) ? k.borde_izq : k.borde_der;
         } else {
             r.c0 = (size_t)j * nx + ie - 1;
             r.c1 = r.c0 + 1;
@@ -134,21 +135,24 @@ __device__ __forceinline__ void edge_inputs(const SV<F>& sv, const double* H, co
     }
 }
 
+// constants of the edge functions in the adjoint's precision F
+template <typename F>
 struct EC {
-    float eps, eps4, g, gp, r, mu, dt, area;
+    F eps, eps4, g, gp, r, mu, dt, area;
 };
 
-__device__ __forceinline__ EC edge_consts(const Consts& k, double dt)
+template <typename F>
+__device__ __forceinline__ EC<F> edge_consts(const Consts& k, double dt)
 {
-    EC c;
-    c.eps = (float)k.eps_h;
+    EC<F> c;
+    c.eps = (F)k.eps_h;
     c.eps4 = c.eps * c.eps * c.eps * c.eps;
-    c.g = (float)k.g;
-    c.gp = (float)(k.g * (1.0 - k.r));
-    c.r = (float)k.r;
-    c.mu = (float)k.mu;
-    c.dt = (float)dt;
-    c.area = (float)k.area;
+    c.g = (F)k.g;
+    c.gp = (F)(k.g * (1.0 - k.r));
+    c.r = (F)k.r;
+    c.mu = (F)k.mu;
+    c.dt = (F)dt;
+    c.area = (F)k.area;
     return c;
 }
 
@@ -161,7 +165,7 @@ __global__ void __launch_bounds__(256) k_recompute(Consts k, SV<F> sv, const dou
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     EdgeRef r;
     if (!edge_ref(k, e, cls, r)) return;
-    const EC c = edge_consts(k, dt_hist[step]);
+    const EC<F> c = edge_consts<F>(k, dt_hist[step]);
     const F a0h1 = w.ACC[r.c0], a0h2 = w.ACC[3 * N + r.c0];
     const F a1h1 = r.interior ? w.ACC[r.c1] : F(0), a1h2 = r.interior ? w.ACC[3 * N + r.c1] : F(0);
     if (cls > 0) {
@@ -248,7 +252,7 @@ __global__ void __launch_bounds__(256) k_edge_adj(Consts k, SV<F> sv, const doub
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     EdgeRef r;
     if (!edge_ref(k, e, cls, r)) return;
-    const EC c = edge_consts(k, dt_hist[step]);
+    const EC<F> c = edge_consts<F>(k, dt_hist[step]);
     F a0h1 = F(0), a0h2 = F(0), a1h1 = F(0), a1h2 = F(0);
     if (cls > 0) {
         const F* P = w.P + (size_t)(cls - 1) * 2 * N;
@@ -302,7 +306,7 @@ __global__ void __launch_bounds__(256) k_edge_adj(Consts k, SV<F> sv, const doub
         w.AB[3 * N + r.c1] += ga1h2;
     } else {
         // the ghost copies cell 0 with its normal discharge times borde
-        const double bd = (double)r.bd;
+        const double bd = (double)(F)r.bd;
         const int nk = r.xedge ? 0 : 1;
         double gq[4];
         for (int q = 0; q < 4; q++) gq[q] = (double)gq0[q] + (double)gq1[q] * (((q & 1) == nk) ? bd : 1.0);
@@ -385,7 +389,7 @@ __global__ void __launch_bounds__(NT) k_adj_fused(Consts k, const float* __restr
     const int bi = i0 - HW, bj = j0 - HW;          // global index of box cell (0, 0)
     const int tid = threadIdx.x;
     const double dt = dt_hist[step];
-    const EC c = edge_consts(k, dt);
+    const EC<float> c = edge_consts<float>(k, dt);
     const float eps = (float)k.eps_h;
 
     // 1. stage the box; cell adjoint of every box cell
