@@ -81,9 +81,25 @@ struct lhysea_batch {
     bool sd_dirty = true;            // device descriptors differ from the host settings
     bool sd_outputs = true;          // device descriptors carry the gauge / eta max / dt outputs
     bool tab_dirty = true;
+    double* stage = nullptr;         // FP64 values of one scenario staged for upload / download
+    size_t stage_n = 0;              // cells the staging buffer holds
 };
 
 namespace {
+
+// staging buffer of at least N cells (6 FP64 planes)
+bool ensure_stage(lhysea_batch* b, size_t N)
+{
+    if (b->stage_n >= N) return true;
+    hipFree(b->stage);
+    b->stage_n = 0;
+    if (hipMalloc(&b->stage, 6 * N * sizeof(double)) != hipSuccess) {
+        b->stage = nullptr;
+        return false;
+    }
+    b->stage_n = N;
+    return true;
+}
 
 void drop_graph(lhysea_batch* b)
 {
@@ -357,6 +373,7 @@ int lhysea_batch_destroy(lhysea_batch* b)
         hipFree(s.list);
     }
     hipFree(b->ctl);
+    hipFree(b->stage);
     hipFree(b->d_sd);
     hipFree(b->d_tab);
     if (b->stream) hipStreamDestroy(b->stream);
@@ -377,22 +394,14 @@ int lhysea_batch_upload(lhysea_batch* b, int32_t sid, const double* W, const dou
     Scen& s = b->sc[sid];
     const size_t N = s.N;
     BHIPCHK(hipStreamSynchronize(b->stream));
-    // split each FP64 value into an FP32 value and its FP32 remainder
-    std::vector<float> buf(12 * N);
-    for (size_t i = 0; i < 6 * N; i++) {
-        const float hi = (float)W[i];
-        buf[i] = hi;
-        buf[6 * N + i] = (float)(W[i] - (double)hi);
-    }
-    BHIPCHK(hipMemcpy(s.W[0], buf.data(), 12 * N * sizeof(float), hipMemcpyHostToDevice));
-    BHIPCHK(hipMemcpy(s.H, H, N * sizeof(double), hipMemcpyHostToDevice));
-    // eta max starts at h1 + h2 - H with time 0 (src/GPU/Problema.cxx:541-546)
-    std::vector<double> em(2 * N);
-    for (size_t i = 0; i < N; i++) {
-        em[2 * i] = W[i] + W[3 * N + i] - H[i];
-        em[2 * i + 1] = 0.0;
-    }
-    BHIPCHK(hipMemcpy(s.emax, em.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice));
+    // the FP64 state staged on the device, split there into FP32 values and
+    // compensations; eta max = (h1 + h2 - H, time 0) from it
+    if (!ensure_stage(b, N)) return bfail(LHYSEA_ERR_GPU_MEMORY, "staging buffer allocation failed");
+    BHIPCHK(hipMemcpyAsync(b->stage, W, 6 * N * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    BHIPCHK(hipMemcpyAsync(s.H, H, N * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    BHIPCHK(f32c_split_state(b->stage, s.W[0], N, b->stream));
+    BHIPCHK(init_eta_max(b->stage, s.H, s.emax, N, b->stream));
+    BHIPCHK(hipStreamSynchronize(b->stream));   // the caller's W and H are read
     Ctl c{};
     c.dt_cur = -1.0;
     BHIPCHK(hipMemcpy(b->ctl + sid, &c, sizeof(Ctl), hipMemcpyHostToDevice));
@@ -410,9 +419,10 @@ int lhysea_batch_download(lhysea_batch* b, int32_t sid, double* W)
     BHIPCHK(hipMemcpy(&c, b->ctl + sid, sizeof(Ctl), hipMemcpyDeviceToHost));
     const Scen& s = b->sc[sid];
     const size_t N = s.N;
-    std::vector<float> buf(12 * N);
-    BHIPCHK(hipMemcpy(buf.data(), s.W[c.step & 1], 12 * N * sizeof(float), hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < 6 * N; i++) W[i] = (double)buf[i] + (double)buf[6 * N + i];
+    if (!ensure_stage(b, N)) return bfail(LHYSEA_ERR_GPU_MEMORY, "staging buffer allocation failed");
+    BHIPCHK(f32c_merge_state(s.W[c.step & 1], b->stage, N, b->stream));
+    BHIPCHK(hipStreamSynchronize(b->stream));
+    BHIPCHK(hipMemcpy(W, b->stage, 6 * N * sizeof(double), hipMemcpyDeviceToHost));
     return LHYSEA_OK;
 }
 

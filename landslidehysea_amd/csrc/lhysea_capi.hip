@@ -51,6 +51,7 @@ struct lhysea_sim {
     Buf32 b32{};
     Ctl* ctl = nullptr;
     double* emax = nullptr;      // 2N interleaved (value, time)
+    double* stage = nullptr;     // FP32C: 6N FP64 values staged for upload / download (allocated on first use)
     bool track_emax = false;
     int* gauges = nullptr;
     int ng = 0;
@@ -294,6 +295,7 @@ int lhysea_destroy(lhysea_sim* s)
     hipFree(s->b32.acc);
     hipFree(s->ctl);
     hipFree(s->emax);
+    hipFree(s->stage);
     hipFree(s->gauges);
     hipFree(s->gseries);
     hipFree(s->dt_hist);
@@ -312,27 +314,29 @@ int lhysea_upload(lhysea_sim* s, const double* W, const double* H)
     const size_t N = s->N;
     HIPCHK(hipStreamSynchronize(s->stream));
     s->cur = 0;
+    // the FP64 state on the device (FP32C: staged, then split into values and
+    // compensations there), and eta max = (h1 + h2 - H, time 0) from it; the
+    // copies go on the handle's stream, ahead of the kernels that read them
+    const double* Wd;
+    const double* Hd;
     if (s->p.precision == LHYSEA_FP64) {
-        HIPCHK(hipMemcpy(s->b64.W[0], W, 6 * N * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(s->b64.H, H, N * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(s->b64.W[0], W, 6 * N * sizeof(double), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(s->b64.H, H, N * sizeof(double), hipMemcpyHostToDevice, s->stream));
+        Wd = s->b64.W[0];
+        Hd = s->b64.H;
     } else {
-        // split each FP64 value into an FP32 value and its FP32 remainder
-        std::vector<float> buf(12 * N);
-        for (size_t i = 0; i < 6 * N; i++) {
-            float hi = (float)W[i];
-            buf[i] = hi;
-            buf[6 * N + i] = (float)(W[i] - (double)hi);
+        if (!s->stage && hipMalloc(&s->stage, 6 * N * sizeof(double)) != hipSuccess) {
+            s->stage = nullptr;
+            return fail(LHYSEA_ERR_GPU_MEMORY, "staging buffer allocation failed");
         }
-        HIPCHK(hipMemcpy(s->b32.W[0], buf.data(), 12 * N * sizeof(float), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(s->b32.H, H, N * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(s->stage, W, 6 * N * sizeof(double), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(s->b32.H, H, N * sizeof(double), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(f32c_split_state(s->stage, s->b32.W[0], N, s->stream));
+        Wd = s->stage;
+        Hd = s->b32.H;
     }
-    // eta max starts at h1 + h2 - H with time 0 (src/GPU/Problema.cxx:541-546)
-    std::vector<double> em(2 * N);
-    for (size_t i = 0; i < N; i++) {
-        em[2 * i] = W[i] + W[3 * N + i] - H[i];
-        em[2 * i + 1] = 0.0;
-    }
-    HIPCHK(hipMemcpy(s->emax, em.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(init_eta_max(Wd, Hd, s->emax, N, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));   // the caller's W and H are read
     Ctl c{};
     c.dt_cur = -1.0;
     HIPCHK(hipMemcpy(s->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice));
@@ -349,9 +353,13 @@ int lhysea_download(lhysea_sim* s, double* W)
     if (s->p.precision == LHYSEA_FP64) {
         HIPCHK(hipMemcpy(W, s->b64.W[s->cur], 6 * N * sizeof(double), hipMemcpyDeviceToHost));
     } else {
-        std::vector<float> buf(12 * N);
-        HIPCHK(hipMemcpy(buf.data(), s->b32.W[s->cur], 12 * N * sizeof(float), hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < 6 * N; i++) W[i] = (double)buf[i] + (double)buf[6 * N + i];
+        if (!s->stage && hipMalloc(&s->stage, 6 * N * sizeof(double)) != hipSuccess) {
+            s->stage = nullptr;
+            return fail(LHYSEA_ERR_GPU_MEMORY, "staging buffer allocation failed");
+        }
+        HIPCHK(f32c_merge_state(s->b32.W[s->cur], s->stage, N, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        HIPCHK(hipMemcpy(W, s->stage, 6 * N * sizeof(double), hipMemcpyDeviceToHost));
     }
     return LHYSEA_OK;
 }

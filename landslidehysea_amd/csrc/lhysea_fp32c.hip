@@ -1102,7 +1102,44 @@ __global__ void k_batch_finalize(const BatchScen* __restrict__ sd)
     }
 }
 
+// State conversion between FP64 planes (6 of N) and the compensated layout
+// (6 value planes, then 6 compensation planes): value = the FP64 number
+// rounded to FP32, compensation = the FP32 rounding of what is left.
+__global__ void __launch_bounds__(256) k_split_state(const double* __restrict__ src, float* __restrict__ W, size_t n6)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n6; i += (size_t)gridDim.x * blockDim.x) {
+        const double x = src[i];
+        const float hi = (float)x;
+        W[i] = hi;
+        W[n6 + i] = (float)(x - (double)hi);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_merge_state(const float* __restrict__ W, double* __restrict__ dst, size_t n6)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n6; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = (double)W[i] + (double)W[n6 + i];
+}
+
 }  // namespace f32
+
+static unsigned stream_blocks(size_t n)
+{
+    const size_t b = (n + 255) / 256;
+    return (unsigned)(b < 8192 ? (b > 0 ? b : 1) : 8192);
+}
+
+hipError_t f32c_split_state(const double* src, float* W, size_t N, hipStream_t s)
+{
+    hipLaunchKernelGGL(f32::k_split_state, dim3(stream_blocks(6 * N)), dim3(256), 0, s, src, W, 6 * N);
+    return hipGetLastError();
+}
+
+hipError_t f32c_merge_state(const float* W, double* dst, size_t N, hipStream_t s)
+{
+    hipLaunchKernelGGL(f32::k_merge_state, dim3(stream_blocks(6 * N)), dim3(256), 0, s, W, dst, 6 * N);
+    return hipGetLastError();
+}
 
 hipError_t f32c_initial_dt(const Consts& k, const float* W, Ctl* ctl, hipStream_t s)
 {

@@ -764,9 +764,27 @@ __global__ void __launch_bounds__(256) k_classify(Consts k, const double* __rest
     if (t == 0 && threadIdx.x == 0) a.ctl->cls_from = a.ctl->step;
 }
 
+// eta max of a fresh state: h1 + h2 - H with time 0 (src/GPU/Problema.cxx:541-546)
+__global__ void __launch_bounds__(256) k_init_eta_max(const double* __restrict__ W, const double* __restrict__ H,
+                                                      double* __restrict__ emax, size_t N)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+        emax[2 * i] = W[i] + W[3 * N + i] - H[i];
+        emax[2 * i + 1] = 0.0;
+    }
+}
+
 }  // namespace f64
 
 // ------------------------------------------------------------------ launchers
+hipError_t init_eta_max(const double* W, const double* H, double* emax, size_t N, hipStream_t s)
+{
+    const size_t b = (N + 255) / 256;
+    hipLaunchKernelGGL(f64::k_init_eta_max, dim3((unsigned)(b < 8192 ? (b > 0 ? b : 1) : 8192)), dim3(256), 0, s, W,
+                       H, emax, N);
+    return hipGetLastError();
+}
+
 hipError_t f64_initial_dt(const Consts& k, const double* W, Ctl* ctl, hipStream_t s)
 {
     const size_t N = (size_t)k.nx * k.ny;

@@ -61,6 +61,8 @@ struct Buf32 {
 };
 
 // FP64 launchers (lhysea_fp64.hip)
+// eta max of a fresh FP64-layout state W (6 planes of N): (h1 + h2 - H, 0) per cell
+hipError_t init_eta_max(const double* W, const double* H, double* emax, size_t N, hipStream_t s);
 hipError_t f64_initial_dt(const Consts& k, const double* W, Ctl* ctl, hipStream_t s);
 hipError_t f64_step_fourpass(const Consts& k, const double* W, double* Wn, const double* H, double* acc,
                              const StepArgs& a, hipStream_t s);
@@ -71,6 +73,9 @@ int f64_tiles(const Consts& k);
 hipError_t f64_classify(const Consts& k, const double* W, const StepArgs& a, hipStream_t s);
 
 // FP32-compensated launchers (lhysea_fp32c.hip)
+// FP64 planes (6 of N) to the compensated layout and back
+hipError_t f32c_split_state(const double* src, float* W, size_t N, hipStream_t s);
+hipError_t f32c_merge_state(const float* W, double* dst, size_t N, hipStream_t s);
 hipError_t f32c_initial_dt(const Consts& k, const float* W, Ctl* ctl, hipStream_t s);
 hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const double* H, const StepArgs& a,
                            hipStream_t s);
