@@ -21,8 +21,11 @@ Checks, bit for bit between paths of the library on the same inputs:
   * a second upload into a handle whose buffers hold sediment everywhere ==
     a fresh handle, both precisions (the sediment-free launch rewrites the
     layer-2 planes until its own writes are two steps back);
-and against the FP64 oracle (C restatement), the emulated FP64 step within
-1e-10 relative L2 over the state.
+and against the FP64 oracle (C restatement): the emulated FP64 step within
+1e-10 relative L2 over the state, and the discrete adjoint (FP64 four-pass and
+FP32C fused reverse steps) against the oracle's central differences with the
+dt sequence frozen (1e-4 and 2e-2, as on the GPU), with binomial checkpoints
+(restores and class rebuilds in workgroup order) equal to keeping every state.
 
 The emulated library is built with -ffp-contract=off, so its arithmetic is not
 that of the gfx950 code object; the GPU tests repeat these checks on the
@@ -153,6 +156,56 @@ def run_cases(lib: str) -> dict:
         a, b = out[1], out[0]
         res[f"run_until_{prec}"] = {"same": bool(np.array_equal(a[0], b[0]) and a[1:] == b[1:]),
                                     "steps": [a[2], a[3]]}
+
+    # discrete adjoint (FP32C: the fused reverse kernel; FP64: the four-pass
+    # reverse step) against central differences of the oracle with the dt
+    # sequence frozen, and binomial checkpointing against keeping every state
+    for prec in (L.FP64, L.FP32C):
+        nsteps = 12
+        sc, W, H, prm = case(24, 18, angle=2.0, semi_axes=(5.0, 4.0), precision=prec)
+        gauges = gauge_cells(sc, 4, seed=3)
+        po = oracle_params(ob, prm)
+        Wt = W.copy()
+        Wt[3] *= 1.1
+        Wt[0] = np.maximum(0.0, (W[0] + W[3]) - Wt[3])
+        obs = ob.run(po, Wt, H, nsteps, gauges=gauges)["gauges"]
+        dts = ob.run(po, W.copy(), H, nsteps, gauges=gauges)["dt_hist"]
+        out = []
+        for nsnap in (4, nsteps + 1):
+            with L.Simulation(prm) as sim:
+                sim.upload(W, H)
+                out.append(sim.adjoint(nsteps, nsnap, gauges, obs))
+        r = out[0]
+
+        def loss(Wx, p=po):
+            g = ob.run_fixed_dt(p, Wx.copy(), H, dts, gauges)
+            return float(((g - obs) ** 2).sum())
+
+        L0 = loss(W)
+        rels, smooth = [], 0
+        rng = np.random.default_rng(0)
+        for c in rng.choice(np.flatnonzero(W[3].ravel() > 5.0), 5, replace=False):
+            j, i = divmod(int(c), prm.nx)
+            for k in (0, 3):
+                eps = 1e-5 * max(1.0, W[k, j, i])
+                Wp, Wm = W.copy(), W.copy()
+                Wp[k, j, i] += eps
+                Wm[k, j, i] -= eps
+                lp, lm = loss(Wp), loss(Wm)
+                fp, fm, fc = (lp - L0) / eps, (L0 - lm) / eps, (lp - lm) / (2 * eps)
+                if abs(fp - fm) <= 1e-3 * abs(fc):
+                    smooth += 1
+                    rels.append(abs(r["grad_W0"][k, j, i] - fc) / abs(fc))
+        pp, pm = oracle_params(ob, prm), oracle_params(ob, prm)
+        for q, sgn in ((pp, 1), (pm, -1)):
+            for a in ("angle1", "angle2", "angle3", "angle4"):
+                setattr(q, a, getattr(q, a) + sgn * 1e-6)
+        fa = (loss(W, pp) - loss(W, pm)) / 2e-6
+        res[f"adjoint_{prec}"] = {
+            "loss_rel": abs(r["loss"] - L0) / L0, "smooth": smooth, "max_rel": max(rels) if rels else None,
+            "angle_rel": abs(r["grad_angle1"] - fa) / abs(fa), "advances": r["advances"],
+            "checkpoints_equal_all_states": bool(np.array_equal(out[0]["grad_W0"], out[1]["grad_W0"])
+                                                 and out[0]["grad_angle1"] == out[1]["grad_angle1"])}
     return res
 
 
@@ -200,6 +253,15 @@ def test_run_until_split_in_workgroup_order(emulated):
     for prec in (1, 0):
         r = emulated[f"run_until_{prec}"]
         assert r["same"] and r["steps"][0] > 5 and r["steps"][1] == 7, r
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-4), (1, 2e-2)])
+def test_adjoint_in_workgroup_order_matches_oracle_fd(emulated, prec, tol):
+    r = emulated[f"adjoint_{prec}"]
+    assert r["loss_rel"] <= (1e-9 if prec == 0 else 5e-2)
+    assert r["smooth"] >= 6 and r["max_rel"] <= tol, r
+    assert r["angle_rel"] <= tol, r
+    assert r["advances"] > 0 and r["checkpoints_equal_all_states"], r
 
 
 if __name__ == "__main__":
