@@ -71,8 +71,15 @@ static void emu_run(dim3 grid, dim3 block, const std::function<void()>& body) {
         emu_nstacks = nt;
     }
     emu_body = &body;
-    for (unsigned by = 0; by < grid.y; by++)
-    for (unsigned bx = 0; bx < grid.x; bx++) {
+    // workgroups in index order, or from the last one down when EMU_ORDER=reverse
+    static const bool reverse = [] {
+        const char* o = std::getenv("EMU_ORDER");
+        return o && std::strcmp(o, "reverse") == 0;
+    }();
+    const unsigned nwg = grid.x * grid.y;
+    for (unsigned q = 0; q < nwg; q++) {
+        const unsigned wg = reverse ? nwg - 1 - q : q;
+        const unsigned bx = wg % grid.x, by = wg / grid.x;
         blockIdx = {bx, by, 0};
         emu_blk = {nt, 0, 0};
         for (unsigned w = 0; w < emu_wav.size(); w++) emu_wav[w] = {std::min(64u, nt - 64 * w), 0, 0};

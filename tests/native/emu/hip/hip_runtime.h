@@ -7,9 +7,10 @@
 // the global address space are ignored, and __all() answers for the calling
 // lane alone (a wavefront's lanes may then take different branches of a
 // wave-uniform choice, which the kernels allow only between forms that give
-// the same result).  Workgroups in index order is the most adverse schedule
-// for protocols between the workgroups of one launch: a workgroup starts only
-// after every lower one has finished.
+// the same result).  Workgroups in index order (or, with EMU_ORDER=reverse,
+// from the last down) is the most adverse schedule for protocols between the
+// workgroups of one launch: a workgroup starts only after every other one
+// before it in that order has finished.
 #pragma once
 #include <atomic>
 #include <cmath>

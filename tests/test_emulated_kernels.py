@@ -2,11 +2,12 @@
 
 tests/native/emu holds a host-side stand-in for <hip/hip_runtime.h> and its
 runtime: the .hip sources compile with g++ into a library with the same C ABI,
-whose kernels run workgroup by workgroup in index order, the lanes of a
-workgroup as cooperative fibers.  That order is the most adverse one for the
-protocols between the workgroups of one launch (the general-tile launch that
-closes a step, the class lists written by one step and read by the next): a
-workgroup starts only after every lower-numbered one has finished, the way a
+whose kernels run workgroup by workgroup in index order (and, in a second
+pass, in reverse index order), the lanes of a workgroup as cooperative
+fibers.  Those orders are the most adverse ones for the protocols between the
+workgroups of one launch (the general-tile launch that closes a step, the
+class lists written by one step and read by the next): a workgroup starts
+only after every one before it in the order has finished, the way a
 workgroup that the GPU schedules late would.  A general-tile launch whose late
 workgroups read the next step's tile list instead of leaving fails the FP32C
 check on the 64 x 48 grid (whose general set grows) and the FP64 check by
@@ -210,16 +211,28 @@ def run_cases(lib: str) -> dict:
 
 
 @pytest.fixture(scope="module")
-def emulated(tmp_path_factory):
+def emulated_lib(tmp_path_factory):
     if shutil.which("g++") is None:
         pytest.skip("needs g++")
     lib = str(tmp_path_factory.mktemp("emu") / "liblhysea_emu.so")
     build_emulated(lib)
-    out = subprocess.run([sys.executable, __file__, lib], capture_output=True, text=True, timeout=600, cwd=ROOT)
+    return lib
+
+
+def cases_in_order(lib, order):
+    env = dict(os.environ, EMU_ORDER=order)
+    out = subprocess.run([sys.executable, __file__, lib], capture_output=True, text=True, timeout=600, cwd=ROOT,
+                         env=env)
     assert out.returncode == 0, out.stdout + out.stderr
     res = json.loads(out.stdout.strip().splitlines()[-1])
-    print(json.dumps(res, indent=1))
+    print(order, json.dumps(res, indent=1))
     return res
+
+
+# every case with the workgroups of a launch in index order and in reverse
+@pytest.fixture(scope="module", params=["forward", "reverse"])
+def emulated(emulated_lib, request):
+    return cases_in_order(emulated_lib, request.param)
 
 
 def test_fp32c_split_step_in_workgroup_order(emulated):
