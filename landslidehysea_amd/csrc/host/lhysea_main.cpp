@@ -326,7 +326,22 @@ int main(int argc, char** argv)
             }
             next_save += pb.tiempo_guardar;
         }
-        rc = check(lhysea_step(sim, 1));
+        if (quiet && pb.tiempo_guardar != 0.0) {
+            // no per-step line: advance to the next save time (or the end) in
+            // one call; the device stops at the first step that starts at or
+            // after it, where the loop above saves.  A save time already
+            // passed (dt above the save interval) takes one step, as the
+            // reference's loop does.
+            double target = (fixed_steps >= 0) ? INFINITY : pb.tiempo_tot;
+            if (pb.tiempo_guardar > 0.0 && next_save < target) target = next_save;
+            const long left = (fixed_steps >= 0) ? fixed_steps - steps : (long)INT32_MAX;
+            int32_t done = 0;
+            rc = (target > t) ? check(lhysea_run_until(sim, target, (int32_t)(left < INT32_MAX ? left : INT32_MAX),
+                                                       &done))
+                              : check(lhysea_step(sim, 1));
+        } else {
+            rc = check(lhysea_step(sim, 1));
+        }
         if (!rc) rc = check(lhysea_get_time(sim, &t, &dt, &steps));
         if (!rc && !quiet) std::printf("Iteracion %3d, deltaT = %e seg, Tiempo = %g seg\n", steps, dt * pb.T, t * pb.T);
     }

@@ -60,13 +60,17 @@ def library_frames(prm, W, H, t_end, t_save):
         return frames, times, sim.eta_max()[0], sim.download()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
-def test_cli_netcdf_run_matches_library(tmp_path, precision):
+# --quiet steps to each save time in one call (run_until); without it the
+# executable steps one at a time and prints a line per step; a save interval
+# shorter than dt saves after every step
+@pytest.mark.parametrize("precision,t_end,t_save,quiet", [("fp64", 12.0, 3.0, True), ("fp32c", 12.0, 3.0, True),
+                                                          ("fp64", 2.0, 0.05, True), ("fp32c", 12.0, 3.0, False)])
+def test_cli_netcdf_run_matches_library(tmp_path, precision, t_end, t_save, quiet):
     from scipy.io import netcdf_file
     sc = S.make_scenario(64, 48, seed=1, semi_axes=(12.0, 9.0))
-    write_case(str(tmp_path), sc, prefix="res", t_end=12.0, t_save=3.0, angle_deg=2.0)
+    write_case(str(tmp_path), sc, prefix="res", t_end=t_end, t_save=t_save, angle_deg=2.0)
     dump = json.loads(cli(["data.dat", "--dump"], tmp_path).stdout)
-    r = cli(["data.dat", "--quiet", "--precision", precision], tmp_path)
+    r = cli(["data.dat", "--precision", precision] + (["--quiet"] if quiet else []), tmp_path)
     say(f"L-HySEA.exe ({precision}) rc={r.returncode}: {r.stdout.strip().splitlines()[-1] if r.stdout else ''}")
     assert r.returncode == 0, r.stderr
     H, _ = S.hmin_shift(sc.H)
