@@ -365,7 +365,8 @@ __global__ void __launch_bounds__(NT) k_adj_fused(Consts k, const float* __restr
                                                   const float* __restrict__ ACC, const double* __restrict__ LIN,
                                                   double* __restrict__ LOUT, double* gmu,
                                                   const double* __restrict__ dt_hist, int step, int tiles_x,
-                                                  int tiles_y)
+                                                  int tiles_y, const int* __restrict__ gauges, int ng,
+                                                  const double* __restrict__ Gk)
 {
     static_assert(TX % 2 == 0 && TY % 2 == 0, "tile origins must stay even");
     constexpr int HW = 2, CW = TX + 2 * HW, RW = TY + 2 * HW;
@@ -585,6 +586,27 @@ __global__ void __launch_bounds__(NT) k_adj_fused(Consts k, const float* __restr
         __syncthreads();
     }
 
+    // the loss gradient of the gauge series at step n (Gk, one per gauge) adds
+    // to the h1 and h2 adjoints of the gauge cells in this tile (eta = h1 + h2
+    // - H); one thread per cell sums the gauges that share it
+    if (Gk) {
+        for (int q = tid; q < ng; q += NT) {
+            const int c = gauges[q], gj = c / nx, gi = c % nx;
+            const int tr = gj - j0, tc = gi - i0;
+            if (tr < 0 || tr >= TY || tc < 0 || tc >= TX) continue;
+            bool first = true;
+            for (int h = 0; h < q; h++)
+                if (gauges[h] == c) first = false;
+            if (!first) continue;
+            double sum = 0.0;
+            for (int h = q; h < ng; h++)
+                if (gauges[h] == c) sum += Gk[h];
+            sL[0][tr][tc] += sum;
+            sL[3][tr][tc] += sum;
+        }
+        __syncthreads();
+    }
+
     // 3. lambda^n of the tile's own cells, d/dmu
     for (int e = tid; e < TX * TY; e += NT) {
         const int tr = e / TX, tc = e % TX;
@@ -734,14 +756,15 @@ hipError_t adj_reverse_step_f32(const Consts& k, const float* W, const double* H
 
 hipError_t adj_reverse_step_f32_fused(const Consts& k, const float* W, const double* H, float* P, float* ACC,
                                       const double* LAMin, double* LAMout, double* gmu, const double* dt_hist,
-                                      int step, hipStream_t s)
+                                      int step, const int* gauges, int ng, const double* Gk, hipStream_t s)
 {
     constexpr int TX = kAdjTX, TY = kAdjTY, NT = 512;
     hipError_t e = f32c_recompute(k, W, H, dt_hist, step, P, ACC, s);
     if (e != hipSuccess) return e;
     const int tiles_x = (k.nx + TX - 1) / TX, tiles_y = (k.ny + TY - 1) / TY;
     hipLaunchKernelGGL((adj::k_adj_fused<TX, TY, NT>), dim3((unsigned)(tiles_x * tiles_y)), dim3(NT), 0, s, k, W, H,
-                       P, ACC, LAMin, LAMout, gmu, dt_hist, step, tiles_x, tiles_y);
+                       P, ACC, LAMin, LAMout, gmu, dt_hist, step, tiles_x, tiles_y, gauges, ng > 0 ? ng : 0,
+                       ng > 0 ? Gk : nullptr);
     return hipGetLastError();
 }
 

@@ -719,17 +719,19 @@ hipError_t hook_reverse(void* c, int k)
     hipError_t e;
     // the state W^k: the working buffer, or its stored slot
     const void* W = a->W ? (const void*)a->W : state_ptr(s);
-    if (s->p.precision == LHYSEA_FP64)
+    // the seeds of the gauge series at step k (rows of G: after step k) join
+    // lambda^k as it is formed
+    const double* Gk = (k >= 1 && a->ng > 0) ? a->G + (size_t)(k - 1) * a->ng : nullptr;
+    if (s->p.precision == LHYSEA_FP64) {
         e = adj_reverse_step_f64(s->k, (const double*)W, s->b64.H, (double*)a->P, (double*)a->ACC, (double*)a->AB,
                                  a->LAM, a->gmu, s->dt_hist, k, s->stream);
-    else {
-        // lambda^{n+1} in LAM, lambda^n into LAM2, then swap
+        if (e == hipSuccess && Gk) e = adj_inject(a->LAM, s->N, a->gauges, a->ng, Gk, s->stream);
+    } else {
+        // lambda^{n+1} in LAM, lambda^n (with the seeds) into LAM2, then swap
         e = adj_reverse_step_f32_fused(s->k, (const float*)W, s->b32.H, (float*)a->P, (float*)a->ACC, a->LAM,
-                                       a->LAM2, a->gmu, s->dt_hist, k, s->stream);
+                                       a->LAM2, a->gmu, s->dt_hist, k, a->gauges, Gk ? a->ng : 0, Gk, s->stream);
         std::swap(a->LAM, a->LAM2);
     }
-    if (e != hipSuccess) return e;
-    if (k >= 1) e = adj_inject(a->LAM, s->N, a->gauges, a->ng, a->G + (size_t)(k - 1) * a->ng, s->stream);
     return e;
 }
 

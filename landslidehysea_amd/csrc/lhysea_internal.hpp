@@ -128,10 +128,11 @@ hipError_t f32c_batch_classify(const BatchScen& d, const BatchScen* sd_dev, int 
 hipError_t adj_reverse_step_f32(const Consts& k, const float* W, const double* H, float* P, float* ACC, float* AB,
                                 double* LAM, double* gmu, const double* dt_hist, int step, hipStream_t s);
 // FP32 reverse step as two launches: the recomputation and one tiled kernel
-// for the cell and edge adjoints (lambda^{n+1} in LAMin, lambda^n to LAMout)
+// for the cell and edge adjoints (lambda^{n+1} in LAMin, lambda^n to LAMout),
+// which also adds the gauge seeds Gk (ng values at the gauge cells; null: none)
 hipError_t adj_reverse_step_f32_fused(const Consts& k, const float* W, const double* H, float* P, float* ACC,
                                       const double* LAMin, double* LAMout, double* gmu, const double* dt_hist,
-                                      int step, hipStream_t s);
+                                      int step, const int* gauges, int ng, const double* Gk, hipStream_t s);
 hipError_t adj_reverse_step_f64(const Consts& k, const double* W, const double* H, double* P, double* ACC,
                                 double* AB, double* LAM, double* gmu, const double* dt_hist, int step, hipStream_t s);
 hipError_t adj_inject(double* LAM, size_t N, const int* gauges, int ng, const double* G, hipStream_t s);
