@@ -85,3 +85,33 @@ def test_fp32c_mass_conservation():
         say(f"layer {k // 3 + 1} mass drift {abs(m1 - m0) / m0:.3e}")
         assert abs(m1 - m0) <= 1e-9 * m0
     assert np.abs(Wg[4]).max() > 0
+
+
+@pytest.mark.parametrize("precision", [L.FP64, L.FP32C])
+def test_state_upload_download_round_trip(precision):
+    """Upload splits FP64 planes into FP32 values and compensations on the
+    device (FP32C) and initialises eta max there; download merges them back.
+    FP64 returns the state unchanged; FP32C returns hi + lo with hi = fl32(x),
+    lo = fl32(x - hi) (numpy performs the same roundings); eta max starts at
+    h1 + h2 - H bit for bit, time 0."""
+    nx, ny = 97, 61
+    rng = np.random.default_rng(12)
+    W = rng.standard_normal((6, ny, nx)) * np.array([1e3, 1.0, 1e-3, 7.0, 1e-9, 3e5]).reshape(6, 1, 1)
+    W[0] = np.abs(W[0])
+    W[3] = np.abs(W[3])
+    W[1, 0, :5] = [0.0, -0.0, 1e-320, -2.5e-45, 3.4e38]
+    H = rng.uniform(-50.0, 400.0, (ny, nx))
+    prm = L.Params(nx=nx, ny=ny, dx=50.0, dy=50.0, precision=precision)
+    with L.Simulation(prm) as sim:
+        sim.upload(np.ascontiguousarray(W), np.ascontiguousarray(H))
+        back = sim.download()
+        sim.track_eta_max(True)
+        emax, etime = sim.eta_max()
+    if precision == L.FP64:
+        assert np.array_equal(back, W)
+    else:
+        hi = W.astype(np.float32)
+        lo = (W - hi.astype(np.float64)).astype(np.float32)
+        assert np.array_equal(back, hi.astype(np.float64) + lo.astype(np.float64))
+        assert np.abs(back - W).max() <= 2.0 ** -46 * np.abs(W).max()
+    assert np.array_equal(emax, W[0] + W[3] - H) and not etime.any()
