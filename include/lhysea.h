@@ -116,7 +116,7 @@ int lhysea_set_kernel(lhysea_sim *s, int32_t mode);
  * and 1-cell halo hold no sediment (h2 = 0, q2 = 0) takes a reduced kernel
  * (single layer in LDS and registers, twice the occupancy), the others the
  * general two-layer tile function.  mode -1 (default): the split launch is
- * used when at least 3000 tiles are sediment-free (its general launch costs
+ * used when at least 1000 tiles are sediment-free (its general launch costs
  * a fixed latency), chosen again every 1024 steps and every run_until
  * chunk; 0: every tile general; 1: always split.  lhysea_tile_classes
  * reports the class tiles, the general ones of the next step and whether
