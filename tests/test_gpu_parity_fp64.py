@@ -161,3 +161,27 @@ def test_shallow_water_one_shot_matches_stepping(oracle):
     oracle.run(po, Wo, H, 25)
     errs, bad = compare(Wf.reshape(6, prm.ny, prm.nx), Wo, H)
     assert not bad, bad
+
+
+@pytest.mark.parametrize("kernel", [L.KERNEL_FUSED, L.KERNEL_FOURPASS])
+def test_pouliquen_law_matches_oracle(oracle, kernel):
+    """Pouliquen friction (friction_law 1, src/GPU/Arista_kernel.cu:20-58;
+    the oracle's law is pinned by closed forms in test_oracle_kat.py) on a
+    flowing slide with four distinct angles: relative L2 <= 1e-10."""
+    from dataclasses import replace
+    steps = 60
+    sc, W, H, prm = case(64, 48, angle=2.0, semi_axes=(12.0, 9.0))
+    prm = replace(prm, friction_law=1, angle_deg=(2.0, 6.0, 3.0, 8.0))
+    Wo = W.copy()
+    oracle.run(oracle_params(oracle, prm), Wo, H, steps)
+    with L.Simulation(prm, kernel=kernel) as sim:
+        sim.upload(W, H)
+        sim.step(steps)
+        Wg = sim.download()
+    errs, bad = compare(Wg, Wo, H)
+    say("pouliquen rel L2:", errs)
+    assert not bad, bad
+    assert np.abs(Wo[4]).max() > 1e-3             # the slide moves
+    Wc = W.copy()                                  # and the law matters: Coulomb at angle 2 differs
+    oracle.run(oracle_params(oracle, replace(prm, friction_law=0)), Wc, H, steps)
+    assert rel_l2(Wc[3], Wo[3]) > 1e-8

@@ -358,3 +358,38 @@ def test_single_layer_dam_break_converges_to_stoker(oracle):
     e2 = _dam_break_error(oracle, 800)
     assert e2 < 0.05, e2
     assert e2 < 0.6 * e1, (e1, e2)
+
+
+def test_pouliquen_friction_closed_forms(oracle):
+    """Pouliquen law (src/GPU/Arista_kernel.cu:20-58) in its three regimes.
+    With h1, h2 >> eps_h the reduced-gravity Froude numbers are
+    Fr_i^2 = u_i^2 / (g (1 - r) h_i) and Fr^2 = Fr1^2 + Fr2^2 + r Fr1^2 Fr2^2:
+    at rest mu = tan3 + (tan4 - tan3) / (1 + h2/L2); for Fr > beta
+    mu = tan1 + (tan2 - tan1) / (1 + beta h2 / (Fr L2)); in between
+    mu_start + (Fr/beta)^chi (mu_stop - mu_start); all times L/H."""
+    import ctypes as C
+    ang = (20.0, 32.0, 22.0, 35.0)
+    r, Lr, Hr = 0.5, 2.0, 4.0
+    p = prm(oracle, friction_law=1, angles_deg=ang, r=r, L=Lr, H=Hr)
+    t1, t2, t3, t4 = [math.tan(math.radians(a)) for a in ang]
+    beta, L2, chi = 0.136, 8.0e-4, 1.0e-3
+    h1, h2 = 3.0, 0.02
+
+    def mu(u1, u2):
+        return oracle.lib().lh_friction(C.byref(p), h1, u1, h2, u2)
+
+    def froude(u1, u2):
+        a, b = u1 * u1 / (G * (1 - r) * h1), u2 * u2 / (G * (1 - r) * h2)
+        return math.sqrt(a + b + r * a * b)
+
+    assert mu(0.0, 0.0) == pytest.approx((t3 + (t4 - t3) / (1 + h2 / L2)) * Lr / Hr, rel=1e-14)
+    F = froude(2.0, 0.5)
+    assert F > beta
+    assert mu(2.0, 0.5) == pytest.approx((t1 + (t2 - t1) / (1 + beta * h2 / (F * L2))) * Lr / Hr, rel=1e-12)
+    F = froude(0.05, 0.002)
+    assert 1e-7 < F < beta
+    stop, start = t1 + (t2 - t1) / (1 + h2 / L2), t3 + (t4 - t3) / (1 + h2 / L2)
+    assert mu(0.05, 0.002) == pytest.approx((start + (F / beta) ** chi * (stop - start)) * Lr / Hr, rel=1e-12)
+    # Coulomb: |tan a1| L/H whatever the flow
+    q = prm(oracle, friction_law=0, angles_deg=ang, L=Lr, H=Hr)
+    assert oracle.lib().lh_friction(C.byref(q), h1, 2.0, h2, 0.5) == pytest.approx(t1 * Lr / Hr, rel=1e-15)
