@@ -90,7 +90,38 @@ struct EdgeFlux {
 struct K32 {
     float eps, eps4, g, gp, r, mu, peso, dtf, area, len_x, len_y;
     float igp, ten_eps, idt_eps, bq;   // 1/gp, 10/eps, 1/(dt+EPS), dt/area
+    int pouliquen;                     // friction law 1: the Pouliquen coefficient replaces mu
+    float tan1, tan2, tan3, tan4, LoverH;
 };
+
+// Pouliquen friction coefficient (src/GPU/Arista_kernel.cu:20-58) in FP32:
+// the reduced-gravity Froude numbers u_i^2 s(h_i) / g' with s the
+// desingularised 1/h, combined as sqrt(Fr1 + Fr2 + (1 - rr) Fr1 Fr2); the
+// stop / start coefficients from the four angles' tangents.
+template <typename KC>
+__device__ __forceinline__ float friction32(const KC& k, float h1, float u1, float h2, float u2)
+{
+    const float x = h1 * k.ten_eps;
+    const float ex = (x < 9.5f) ? fexp(-(x * x)) : 0.0f;
+    const float rr = 1.0f - k.r * (1.0f - ex);
+    const float igp = rcp(k.g * rr);
+    const float fr1 = u1 * u1 * sdes(h1, k.eps, k.eps4) * igp;
+    const float fr2 = u2 * u2 * sdes(h2, k.eps, k.eps4) * igp;
+    const float fr = fsqrt(fr1 + fr2 + (1.0f - rr) * fr1 * fr2);
+    const float beta = 0.136f, L2 = 8.0e-4f, chi = 1.0e-3f;
+    const float dstop = k.tan2 - k.tan1, dstart = k.tan4 - k.tan3;
+    const float den = rcp(1.0f + h2 * (1.0f / L2));
+    float muf;
+    if (fr > beta)
+        muf = k.tan1 + dstop * rcp(1.0f + beta * h2 * rcp(fr * L2));
+    else if (fabsf(fr) < LH_EPSF)
+        muf = k.tan3 + dstart * den;
+    else {
+        const float mustop = k.tan1 + dstop * den, mustart = k.tan3 + dstart * den;
+        muf = mustart + fpow(fr * (1.0f / beta), chi) * (mustop - mustart);
+    }
+    return muf * k.LoverH;
+}
 
 // One edge (src/GPU/Arista_kernel.cu:324-645) with FP32 arithmetic and FP64
 // hydrostatic terms.  W0/W1: FP32 states (ghost when !interior); hd*: full
@@ -134,7 +165,8 @@ __device__ __forceinline__ bool edge(const K32& k, const float* W0, const float*
     float ex = 0.0f;
     if (x < 9.5f) ex = fexp(-(x * x));         // exp(-x^2) is 0 in FP32 beyond x = 9.4
     float fsc = 1.0f - r * (1.0f - ex);
-    float sc = fsc * k.mu * g * h2ij;
+    const float muc = k.pouliquen ? friction32(k, h1ij, u1ij, h2ij, u2ij) : k.mu;
+    float sc = fsc * muc * g * h2ij;
     bool coul = fabsf(h2ij * u2ij) < k.peso * sc * k.dtf;
     float t23 = coul ? (g * h2ij * r * deta1) : tp3;
 
@@ -436,6 +468,8 @@ __device__ __forceinline__ bool negative(Pair a) { return a.hi < 0.0f || (a.hi =
 struct KCell {
     float eps, eps4, r, g, mu, mfc, mf0, mfs, vmax1, vmax2, dtf, valf;
     float ieps, ten_eps;
+    int pouliquen;
+    float tan1, tan2, tan3, tan4, LoverH;
 };
 
 // Cell update (src/GPU/Volumen_kernel.cu:7-180, 231-284) on compensated
@@ -529,7 +563,13 @@ __device__ __forceinline__ void cell_update(const KCell& k, Pair* o, const float
         float ex = 0.0f;
         if (x < 9.5f) ex = fexp(-(x * x));
         float fsc = 1.0f - k.r * (1.0f - ex);
-        float sc = fsc * k.mu * k.g * n[3].hi * dt;
+        float muc = k.mu;
+        if (k.pouliquen) {
+            const float u1 = fsqrt(n[1].hi * n[1].hi + n[2].hi * n[2].hi) * sdes(n[0].hi, k.eps, k.eps4);
+            const float u2 = fsqrt(n[4].hi * n[4].hi + n[5].hi * n[5].hi) * sdes(n[3].hi, k.eps, k.eps4);
+            muc = friction32(k, n[0].hi, u1, n[3].hi, u2);
+        }
+        float sc = fsc * muc * k.g * n[3].hi * dt;
         float nq = fsqrt(n[4].hi * n[4].hi + n[5].hi * n[5].hi);
         float f = ((nq + sc >= LH_EPSF) && (n[3].hi > 0.0f) && (nq >= sc)) ? nq * rcp(nq + sc) : 0.0f;
         if (f != 1.0f) { n[4] = scale(n[4], f); n[5] = scale(n[5], f); }

@@ -61,6 +61,12 @@ __device__ __forceinline__ K32 make_k32(const Consts& k, double dt)
     c.ten_eps = (float)(10.0 / k.eps_h);
     c.idt_eps = (float)(1.0 / ((double)c.dtf + (double)LH_EPSF));
     c.bq = (float)(dt / k.area);
+    c.pouliquen = k.friction_law == 1;
+    c.tan1 = (float)k.tan1;
+    c.tan2 = (float)k.tan2;
+    c.tan3 = (float)k.tan3;
+    c.tan4 = (float)k.tan4;
+    c.LoverH = (float)k.LoverH;
     return c;
 }
 
@@ -81,6 +87,12 @@ __device__ __forceinline__ KCell make_kcell(const Consts& k, const K32& kc, doub
     kk.valf = (float)(dt / k.area);
     kk.ieps = (float)(1.0 / k.eps_h);
     kk.ten_eps = kc.ten_eps;
+    kk.pouliquen = kc.pouliquen;
+    kk.tan1 = kc.tan1;
+    kk.tan2 = kc.tan2;
+    kk.tan3 = kc.tan3;
+    kk.tan4 = kc.tan4;
+    kk.LoverH = kc.LoverH;
     return kk;
 }
 

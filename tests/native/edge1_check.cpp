@@ -5,7 +5,8 @@
 //    near-dry depths, steps in the bottom, interior and ghost (wall / open)
 //    edges in x and y;
 //  * cell_update1() must give cell_update()'s layer-1 state and cell_update()
-//    a zero layer-2 state, on random water-only cells with zero layer-2 sums;
+//    a zero layer-2 state, on random water-only cells with zero layer-2 sums
+//    (both FP32 checks under the Coulomb and the Pouliquen law);
 //  * the FP64 edge1() must give the FP64 edge()'s layer-1 fluxes and wave
 //    speed, and the FP64 cell_update1() cell_update()'s state, for the
 //    Coulomb and the Pouliquen law.
@@ -42,6 +43,8 @@ int main(int argc, char** argv)
         k.ten_eps = 10.0f / k.eps;
         k.idt_eps = (float)(1.0 / ((double)k.dtf + (double)LH_EPSF));
         k.bq = (float)(k.dtf / (double)k.area);
+        k.pouliquen = U(rng) < 0.3;
+        k.tan1 = 0.15f; k.tan2 = 0.25f; k.tan3 = 0.17f; k.tan4 = 0.3f; k.LoverH = 1.0f;
         const double H0 = 400 * U(rng);
         double H1 = std::max(0.0, H0 + (U(rng) < 0.3 ? 0.0 : 20 * (U(rng) - 0.5)));
         auto depth = [&](double H) {
@@ -108,6 +111,8 @@ int main(int argc, char** argv)
         k.valf = (float)(k.dtf / 3091.0);
         k.ieps = 1.0f / k.eps;
         k.ten_eps = 10.0f / k.eps;
+        k.pouliquen = U(rng) < 0.3;
+        k.tan1 = 0.15f; k.tan2 = 0.25f; k.tan3 = 0.17f; k.tan4 = 0.3f; k.LoverH = 1.0f;
         const double r = U(rng);
         const double h = r < 0.1 ? 0.0 : r < 0.3 ? 0.02 * U(rng) : 300 * U(rng);
         Pair o[6], p[6];

@@ -115,3 +115,28 @@ def test_state_upload_download_round_trip(precision):
         assert np.array_equal(back, hi.astype(np.float64) + lo.astype(np.float64))
         assert np.abs(back - W).max() <= 2.0 ** -46 * np.abs(W).max()
     assert np.array_equal(emax, W[0] + W[3] - H) and not etime.any()
+
+
+def test_fp32c_pouliquen_tracks_fp64_oracle(oracle):
+    """The Pouliquen law (friction_law 1) in the FP32C step, four distinct
+    angles on a flowing slide: the same bars as the Coulomb case."""
+    from dataclasses import replace
+    nx, ny, steps = 96, 72, 150
+    sc, W, H, prm = case(nx, ny, precision=L.FP32C, angle=2.0, semi_axes=(nx / 6, ny / 6), thickness=(40.0, 40.0))
+    prm = replace(prm, friction_law=1, angle_deg=(2.0, 6.0, 3.0, 8.0))
+    gauges = gauge_cells(sc, 6, seed=11)
+    Wo = W.copy()
+    res = oracle.run(oracle_params(oracle, prm), Wo, H, steps, gauges=gauges)
+    with L.Simulation(prm) as sim:
+        sim.upload(W, H)
+        sim.set_gauges(gauges, steps)
+        sim.step(steps)
+        Wg = sim.download()
+        gs = sim.gauges()
+    rest = float((W[0] + W[3] - H)[W[0] > 1.0].mean())
+    eta_err = signal_err(Wg[0] + Wg[3] - H, Wo[0] + Wo[3] - H, rest)
+    g_err = signal_err(gs, res["gauges"], rest)
+    h2_err = rel_l2(Wg[3], Wo[3])
+    say(f"pouliquen fp32c: eta signal {eta_err:.3e} gauges {g_err:.3e} h2 {h2_err:.3e}")
+    assert np.abs(Wo[4]).max() > 1e-3
+    assert eta_err <= 1e-4 and g_err <= 1e-4 and h2_err <= 1e-4
