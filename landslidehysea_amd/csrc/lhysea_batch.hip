@@ -314,8 +314,9 @@ int lhysea_batch_create(const lhysea_params* p, int32_t n, lhysea_batch** out)
         if (q.precision != LHYSEA_FP32C) return bfail(LHYSEA_ERR_ARGUMENT, "batches run the FP32C path");
         if (!(q.dx > 0) || !(q.dy > 0) || !(q.eps_h > 0) || !(q.H > 0))
             return bfail(LHYSEA_ERR_ARGUMENT, "dx, dy, eps_h and H must be positive");
-        if (q.friction_law != 0 || q.peso != 1.0 || q.beta != 1.0)
-            return bfail(LHYSEA_ERR_ARGUMENT, "the FP32C path implements the Coulomb law with peso = beta = 1");
+        if (q.friction_law < 0 || q.friction_law > 1 || q.peso != 1.0 || q.beta != 1.0)
+            return bfail(LHYSEA_ERR_ARGUMENT,
+                         "the FP32C path implements the Coulomb and Pouliquen laws with peso = beta = 1");
     }
     if (lhysea_device_probe() != 0) return bfail(LHYSEA_ERR_NO_DEVICE, "no usable HIP device");
     lhysea_batch* b = new (std::nothrow) lhysea_batch;

@@ -49,7 +49,7 @@ def test_batch_rejects_invalid_scenarios_before_touching_a_device():
     with pytest.raises(solver.LhyseaError) as e:
         solver.Batch([p])
     assert e.value.code == 3 and "FP32C" in str(e.value)
-    q = solver.Params(nx=8, ny=8, dx=1.0, dy=1.0, precision=solver.FP32C, friction_law=1)
+    q = solver.Params(nx=8, ny=8, dx=1.0, dy=1.0, precision=solver.FP32C, friction_law=2)
     with pytest.raises(solver.LhyseaError) as e:
         solver.Batch([q])
     assert e.value.code == 3

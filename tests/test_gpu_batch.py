@@ -182,3 +182,23 @@ def test_batch_timing_reports_time():
         ms = b.time_steps(20)
         assert ms > 0 and math.isfinite(ms)
         assert b.time(0)[2] == 20
+
+
+def test_batch_pouliquen_equals_single_runs():
+    """Scenarios under the Pouliquen law (four angles) beside a Coulomb one in
+    one batch equal their single-grid runs bit for bit."""
+    from dataclasses import replace
+    cases = build(SPECS[:3])
+    cases[0] = cases[0][:3] + (replace(cases[0][3], friction_law=1, angle_deg=(2.0, 6.0, 3.0, 8.0)),)
+    cases[2] = cases[2][:3] + (replace(cases[2][3], friction_law=1, angle_deg=(1.0, 5.0, 2.0, 7.0)),)
+    steps = 40
+    with L.Batch([c[3] for c in cases]) as b:
+        for i, (sc, W, H, prm) in enumerate(cases):
+            b.upload(i, W, H)
+        b.step(steps)
+        got = [(b.download(i), b.time(i)) for i in range(len(cases))]
+    for i, (sc, W, H, prm) in enumerate(cases):
+        ref = single(sc, W, H, prm, steps)
+        say(f"scenario {i} law {prm.friction_law}: t {got[i][1][0]:.6f}")
+        assert got[i][1] == ref["time"] and same(got[i][0], ref["W"])
+    assert np.abs(got[0][0][4]).max() > 0.0
