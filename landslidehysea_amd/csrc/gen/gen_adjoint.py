@@ -28,6 +28,13 @@ def sdes(h, eps, eps4):
     return select(big, rcp(select(big, h, 1.0)), SQRT2 * h * rsq(h2 * h2 + eps4))
 
 
+def hsdes(h, eps, eps4):
+    """h * sdes(h): 1 for h >= eps, else sqrt2 h^2 / sqrt(h^4 + eps^4)."""
+    big = h >= eps
+    h2 = h * h
+    return select(big, 1.0, SQRT2 * h2 * rsq(h2 * h2 + eps4))
+
+
 def sort_net(D):
     x, y, z, w = D
 
@@ -225,12 +232,13 @@ def build_cell():
         aux = exp(-dt * rcp(z2 * z2 + EPS))
         n[qk] = n[qk] * aux
         n[qk + 1] = n[qk + 1] * aux
+    # speed caps: q <- (q s(h)) (vmax / u) h (src/GPU/Volumen_kernel.cu:150-179)
     for hk, qk, vm in ((0, 1, vmax1), (3, 4, vmax2)):
         s = sdes(n[hk], eps, eps4)
         ux, uy = n[qk] * s, n[qk + 1] * s
         u = sqrt(ux * ux + uy * uy)
         cap = (vm > 0.0) & (u > vm)
-        f = vm * rcp(select(cap, u, one))
+        f = vm * rcp(select(cap, u, one)) * hsdes(n[hk], eps, eps4)
         n[qk] = select(cap, n[qk] * f, n[qk])
         n[qk + 1] = select(cap, n[qk + 1] * f, n[qk + 1])
     # disImplicita

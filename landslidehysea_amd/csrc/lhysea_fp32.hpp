@@ -42,6 +42,15 @@ __device__ __forceinline__ float sdes(float h, float eps, float eps4)
     return SQRT2F * h * rsq(h2 * h2 + m2 * m2);
 }
 
+// h * s(h): the factor by which q -> U(h,q) * h shrinks a discharge; exactly 1
+// for h >= eps, sqrt2 h^2 / sqrt(h^4 + eps^4) < 1 below
+__device__ __forceinline__ float hsdes(float h, float eps, float eps4)
+{
+    if (h >= eps) return 1.0f;
+    float h2 = h * h;
+    return SQRT2F * h2 * rsq(h2 * h2 + eps4);
+}
+
 // Two-layer eigenvalue estimate + the reference's compare-swap network
 // (src/GPU/Arista_kernel.cu:152-237), FLAG 0: (h1,u1,h2,u2), FLAG 1: (h1,q1,h2,q2).
 // s0, s2: s(f0), s(f2) of the layers (FLAG 1 only); igp = 1/gp
@@ -506,15 +515,22 @@ __device__ __forceinline__ void cell_update(const KCell& k, Pair* o, const float
             n[5] = scale(n[5], aux);
         }
     }
+    // speed caps: q <- (q s(h)) (vmax / u) h, i.e. q scaled by (vmax / u) h s(h)
     if (k.vmax1 > 0.0f) {
         float h = n[0].hi, s = sdes(h, eps, eps4);
         float ux = n[1].hi * s, uy = n[2].hi * s, u = fsqrt(ux * ux + uy * uy);
-        if (u > k.vmax1) { float c = k.vmax1 * rcp(u); n[1] = scale(n[1], c); n[2] = scale(n[2], c); }
+        if (u > k.vmax1) {
+            float c = k.vmax1 * rcp(u) * hsdes(h, eps, eps4);
+            n[1] = scale(n[1], c); n[2] = scale(n[2], c);
+        }
     }
     if (k.vmax2 > 0.0f) {
         float h = n[3].hi, s = sdes(h, eps, eps4);
         float ux = n[4].hi * s, uy = n[5].hi * s, u = fsqrt(ux * ux + uy * uy);
-        if (u > k.vmax2) { float c = k.vmax2 * rcp(u); n[4] = scale(n[4], c); n[5] = scale(n[5], c); }
+        if (u > k.vmax2) {
+            float c = k.vmax2 * rcp(u) * hsdes(h, eps, eps4);
+            n[4] = scale(n[4], c); n[5] = scale(n[5], c);
+        }
     }
 
     // disImplicita: three sequential, non-exclusive cases, each linear in q
@@ -607,7 +623,10 @@ __device__ __forceinline__ void cell_update1(const KCell& k, Pair* o, const floa
     if (k.vmax1 > 0.0f) {
         float h = n[0].hi, s = sdes(h, eps, eps4);
         float ux = n[1].hi * s, uy = n[2].hi * s, u = fsqrt(ux * ux + uy * uy);
-        if (u > k.vmax1) { float c = k.vmax1 * rcp(u); n[1] = scale(n[1], c); n[2] = scale(n[2], c); }
+        if (u > k.vmax1) {
+            float c = k.vmax1 * rcp(u) * hsdes(h, eps, eps4);
+            n[1] = scale(n[1], c); n[2] = scale(n[2], c);
+        }
     }
     {
         const float h1 = n[0].hi;

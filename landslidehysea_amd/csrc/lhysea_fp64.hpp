@@ -3,9 +3,12 @@
 // Every expression keeps the reference's evaluation order so that, compiled
 // without contraction (the including file sets `#pragma clang fp contract(off)`),
 // the FP64 path rounds one operation at a time like the FP64 oracle.
-// powf(x,4) of the reference is (x*x)*(x*x); tan() is hoisted to the host.
+// powf(x,4) of the reference is (x*x)*(x*x); tan() is hoisted to the host;
+// expf and powf are the deterministic dm:: functions (lhysea_dmath.hpp), whose
+// restatement in the oracle gives the same bits.
 #pragma once
 #include "lhysea_common.hpp"
+#include "lhysea_dmath.hpp"
 
 namespace lhysea {
 namespace f64 {
@@ -23,7 +26,7 @@ __device__ __forceinline__ double friction(const Consts& k, double h1ij, double 
 {
     if (k.friction_law == 0) return k.mu;
     double x = 10.0 * h1ij / k.eps_h;
-    double rr = 1.0 - k.r * (1.0 - exp(-(x * x)));
+    double rr = 1.0 - k.r * (1.0 - dm::exp(-(x * x)));
     double gp = k.g * rr;
     double fr1 = LH_SQRT2 * (u1ij * u1ij) * h1ij / (gp * sqrt(p4(h1ij) + p4(fmax(h1ij, k.eps_h))));
     double fr2 = LH_SQRT2 * (u2ij * u2ij) * h2ij / (gp * sqrt(p4(h2ij) + p4(fmax(h2ij, k.eps_h))));
@@ -34,7 +37,7 @@ __device__ __forceinline__ double friction(const Consts& k, double h1ij, double 
     double muf;
     if (fr > beta) muf = k.tan1 + (k.tan2 - k.tan1) / (1.0 + beta * h2ij / (fr * L2));
     else if (fabs(fr) < LH_EPS) muf = k.tan3 + (k.tan4 - k.tan3) / (1.0 + h2ij / L2);
-    else muf = mustart + pow(fr / beta, chi) * (mustop - mustart);
+    else muf = mustart + dm::powpos(fr / beta, chi) * (mustop - mustart);
     return muf * k.LoverH;
 }
 
@@ -129,7 +132,7 @@ __device__ __forceinline__ bool edge(const Consts& k, const double* W0, const do
     double tp3 = g * h2ij * ((1 - r) * deta2 + r * deta1);
     double muc = friction(k, h1ij, u1ij, h2ij, u2ij);
     double x = 10.0 * h1ij / eps;
-    double fsc = 1.0 - r * (1.0 - exp(-(x * x)));
+    double fsc = 1.0 - r * (1.0 - dm::exp(-(x * x)));
     double sc = fsc * muc * g * h2ij;
     bool coul = fabs(h2ij * u2ij) < k.peso * sc * dt;
     double t20 = 0.0, t21 = tp1, t22 = 0.0, t23 = coul ? (g * h2ij * r * deta1) : tp3;
@@ -413,10 +416,10 @@ __device__ __forceinline__ void cell_update(const Consts& k, const double* o, co
     if (n10 < 0.0) n10 = 0.0;
     if (n20 < 0.0) n20 = 0.0;
     double aux0 = 1.0 / (p4(n10 / eps) + LH_EPS);
-    double aux = exp(-dt * aux0);
+    double aux = dm::exp(-dt * aux0);
     n11 *= aux; n12 *= aux;
     aux0 = 1.0 / (p4(n20 / eps) + LH_EPS);
-    aux = exp(-dt * aux0);
+    aux = dm::exp(-dt * aux0);
     n21 *= aux; n22 *= aux;
     if (k.vmax1 > 0.0) {
         double h = n10, hm = sqrt(p4(h) + p4(fmax(h, eps)));
@@ -451,7 +454,7 @@ __device__ __forceinline__ void cell_update(const Consts& k, const double* o, co
         if ((h1 > 0) && (h2 > 0)) {
             double c1 = dt * LH_SQRT2 * h2 * hmod / hmodm * k.mfc * du;
             double c2 = dt * r * LH_SQRT2 * h1 * hmod / hmodm * k.mfc * du;
-            double c3 = dt * g * k.mfs * k.mfs * u2 / (pow(h2, 4.0 / 3.0) + LH_EPS);
+            double c3 = dt * g * k.mfs * k.mfs * u2 / (dm::pow43(h2) + LH_EPS);
             double det = 1.0 / ((1.0 + c1) * (1.0 + c2 + c3) - c1 * c2);
             n11 = h1 * (u1x * (1.0 + c2 + c3) + c1 * u2x) * det;
             n21 = h2 * (u2x * (1.0 + c1) + c2 * u1x) * det;
@@ -461,14 +464,14 @@ __device__ __forceinline__ void cell_update(const Consts& k, const double* o, co
         if ((h1 > 0) && (h2 < eps)) {
             u1x = LH_SQRT2 * n11 * h1 / h1m;
             u1y = LH_SQRT2 * n12 * h1 / h1m;
-            double c1 = dt * g * k.mf0 * k.mf0 * u1 / (pow(h1, 4.0 / 3.0) + LH_EPS);
+            double c1 = dt * g * k.mf0 * k.mf0 * u1 / (dm::pow43(h1) + LH_EPS);
             n11 = h1 * u1x / (1.0 + c1);
             n12 = h1 * u1y / (1.0 + c1);
         }
         if ((h2 > 0) && (h1 < eps)) {
             u2x = LH_SQRT2 * n21 * h2 / h2m;
             u2y = LH_SQRT2 * n22 * h2 / h2m;
-            double c1 = dt * g * k.mfs * k.mfs * u2 / (pow(h2, 4.0 / 3.0) + LH_EPS);
+            double c1 = dt * g * k.mfs * k.mfs * u2 / (dm::pow43(h2) + LH_EPS);
             n21 = h2 * u2x / (1.0 + c1);
             n22 = h2 * u2y / (1.0 + c1);
         }
@@ -477,7 +480,7 @@ __device__ __forceinline__ void cell_update(const Consts& k, const double* o, co
     // coulomb
     {
         double x = 10.0 * n10 / eps;
-        double fsc = 1.0 - r * (1.0 - exp(-(x * x)));
+        double fsc = 1.0 - r * (1.0 - dm::exp(-(x * x)));
         double muc;
         if (k.friction_law == 0) {
             muc = k.mu;
@@ -513,7 +516,7 @@ __device__ __forceinline__ void cell_update1(const Consts& k, const double* o, c
     // filtroEstado
     if (n10 < 0.0) n10 = 0.0;
     double aux0 = 1.0 / (p4(n10 / eps) + LH_EPS);
-    double aux = exp(-dt * aux0);
+    double aux = dm::exp(-dt * aux0);
     n11 *= aux; n12 *= aux;
     if (k.vmax1 > 0.0) {
         double h = n10, hm = sqrt(p4(h) + p4(fmax(h, eps)));
@@ -532,7 +535,7 @@ __device__ __forceinline__ void cell_update1(const Consts& k, const double* o, c
         if (h1 > 0) {
             double u1x = LH_SQRT2 * n11 * h1 / h1m;
             double u1y = LH_SQRT2 * n12 * h1 / h1m;
-            double c1 = dt * g * k.mf0 * k.mf0 * u1 / (pow(h1, 4.0 / 3.0) + LH_EPS);
+            double c1 = dt * g * k.mf0 * k.mf0 * u1 / (dm::pow43(h1) + LH_EPS);
             n11 = h1 * u1x / (1.0 + c1);
             n12 = h1 * u1y / (1.0 + c1);
         }

@@ -51,36 +51,42 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        P = C.POINTER(LhParams)
-        L.lh_edge.argtypes = [P, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
-                              C.c_double, _dp, _dp, C.c_int]
-        L.lh_edge.restype = C.c_int
-        L.lh_edge_dt0.argtypes = [P, _dp, _dp, C.c_double, C.c_double, C.c_double]
-        L.lh_edge_dt0.restype = C.c_double
-        L.lh_eig.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
-        L.lh_flux1d.argtypes = [_dp, C.c_double, _dp]
-        L.lh_pvm_coeffs.argtypes = [C.c_double] * 7 + [_dp]
-        L.lh_friction.argtypes = [P, C.c_double, C.c_double, C.c_double, C.c_double]
-        L.lh_friction.restype = C.c_double
-        L.lh_filtro.argtypes = [P, _dp, _dp, C.c_double]
-        L.lh_dis_implicita.argtypes = [P, _dp, _dp, _dp, _dp, C.c_double]
-        L.lh_coulomb.argtypes = [P, _dp, _dp, C.c_double]
-        L.lh_cell_update.argtypes = [P, _dp, _dp, C.c_double, _dp]
-        L.lh_cell_update.restype = C.c_double
-        L.lh_oracle_initial_dt.argtypes = [P, _dp, _dp]
-        L.lh_oracle_initial_dt.restype = C.c_double
-        L.lh_oracle_step.argtypes = [P, _dp, _dp, C.c_double, _dp]
-        L.lh_oracle_step.restype = C.c_double
-        L.lh_oracle_run.argtypes = [P, _dp, _dp, C.c_int, _dp, _dp, _dp, _ip, C.c_int, _dp, _dp]
-        L.lh_oracle_run.restype = C.c_int
-        L.lh_oracle_run_fixed_dt.argtypes = [P, _dp, _dp, C.c_int, _dp, _ip, C.c_int, _dp]
-        L.lh_oracle_run_fixed_dt.restype = C.c_int
-        L.lh_oracle_set_threads.argtypes = [C.c_int]
-        L.lh_oracle_get_threads.restype = C.c_int
-        L.lh_oracle_wtime.restype = C.c_double
-        _lib = L
+        _lib = open_library(LIB_PATH)
     return _lib
+
+
+def open_library(path: str):
+    """Load an oracle build at path with its entry points typed (tests load
+    variants of the oracle with a planted mistake this way)."""
+    L = C.CDLL(path)
+    P = C.POINTER(LhParams)
+    L.lh_edge.argtypes = [P, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                          C.c_double, _dp, _dp, C.c_int]
+    L.lh_edge.restype = C.c_int
+    L.lh_edge_dt0.argtypes = [P, _dp, _dp, C.c_double, C.c_double, C.c_double]
+    L.lh_edge_dt0.restype = C.c_double
+    L.lh_eig.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
+    L.lh_flux1d.argtypes = [_dp, C.c_double, _dp]
+    L.lh_pvm_coeffs.argtypes = [C.c_double] * 7 + [_dp]
+    L.lh_friction.argtypes = [P, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.lh_friction.restype = C.c_double
+    L.lh_filtro.argtypes = [P, _dp, _dp, C.c_double]
+    L.lh_dis_implicita.argtypes = [P, _dp, _dp, _dp, _dp, C.c_double]
+    L.lh_coulomb.argtypes = [P, _dp, _dp, C.c_double]
+    L.lh_cell_update.argtypes = [P, _dp, _dp, C.c_double, _dp]
+    L.lh_cell_update.restype = C.c_double
+    L.lh_oracle_initial_dt.argtypes = [P, _dp, _dp]
+    L.lh_oracle_initial_dt.restype = C.c_double
+    L.lh_oracle_step.argtypes = [P, _dp, _dp, C.c_double, _dp]
+    L.lh_oracle_step.restype = C.c_double
+    L.lh_oracle_run.argtypes = [P, _dp, _dp, C.c_int, _dp, _dp, _dp, _ip, C.c_int, _dp, _dp]
+    L.lh_oracle_run.restype = C.c_int
+    L.lh_oracle_run_fixed_dt.argtypes = [P, _dp, _dp, C.c_int, _dp, _ip, C.c_int, _dp]
+    L.lh_oracle_run_fixed_dt.restype = C.c_int
+    L.lh_oracle_set_threads.argtypes = [C.c_int]
+    L.lh_oracle_get_threads.restype = C.c_int
+    L.lh_oracle_wtime.restype = C.c_double
+    return L
 
 
 def ptr(a):

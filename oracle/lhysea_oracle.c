@@ -23,7 +23,11 @@
  * (src/GPU/Arista_kernel.cu:20-58) is available as friction_law == 1.
  *
  * powf(x,4) and powf(x,2) of the reference are evaluated as (x*x)*(x*x) and
- * x*x; tan(delta) is a per-run constant evaluated once on the host.
+ * x*x; tan(delta) is a per-run constant evaluated once on the host.  expf and
+ * powf are evaluated by lh_exp, lh_pow43 and lh_powpos below: FP64 functions
+ * built from IEEE-exact operations only, so that a device restating the same
+ * evaluation reproduces the oracle to the last bit (a platform libm's exp and
+ * pow differ between machines in the last bit).
  * Build with -ffp-contract=off so that no fused multiply-add changes the
  * rounding of the restated expressions.
  */
@@ -54,6 +58,81 @@ typedef struct {
 } lh_params;
 
 static inline double p4(double x) { double x2 = x * x; return x2 * x2; }
+
+/*
+ * Deterministic FP64 elementary functions (+, -, *, /, floor, frexp, ldexp only):
+ *  lh_exp:  x = k ln2 + r, ln2 split into a 32-bit head and a tail (Cody and
+ *           Waite, Software Manual for the Elementary Functions, 1980),
+ *           |r| <= ln2/2, exp(r) by its Taylor series to r^13/13!, times 2^k;
+ *           results below 2^-1020 are 0.
+ *  lh_log:  x = m 2^e, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh((m-1)/(m+1))
+ *           by the odd series to s^23/23.
+ *  lh_cbrt: x = m 2^(3q), m in [1/16, 1/2), Newton's iteration for y^3 = m
+ *           from the chord of cbrt on that interval, six steps.
+ * Checked on the host against the C library (a few ulp) in tests/.
+ */
+#define LH_LN2_HI 6.93147180369123816490e-01
+#define LH_LN2_LO 1.90821492927058770002e-10
+
+double lh_exp(double x)
+{
+    double k, r, p;
+    if (x != x) return x;
+    if (x > 709.782712893383973096) return HUGE_VAL;
+    if (x < -707.0) return 0.0;
+    k = floor(x * 1.44269504088896338700e+00 + 0.5);
+    r = (x - k * LH_LN2_HI) - k * LH_LN2_LO;
+    p = 1.6059043836821614599e-10;                 /* 1/13! */
+    p = p * r + 2.0876756987868098979e-09;         /* 1/12! */
+    p = p * r + 2.5052108385441718775e-08;
+    p = p * r + 2.7557319223985890653e-07;
+    p = p * r + 2.7557319223985890653e-06;
+    p = p * r + 2.4801587301587301587e-05;
+    p = p * r + 1.9841269841269841270e-04;
+    p = p * r + 1.3888888888888888889e-03;
+    p = p * r + 8.3333333333333333333e-03;
+    p = p * r + 4.1666666666666666667e-02;
+    p = p * r + 1.6666666666666666667e-01;         /* 1/3! */
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    return ldexp(p, (int)k);
+}
+
+double lh_log(double x)
+{
+    int e, i;
+    double m, s, s2, t, de;
+    if (!(x > 0.0)) return (x == 0.0) ? -HUGE_VAL : (x - x) / (x - x);
+    if (x == HUGE_VAL) return x;
+    m = frexp(x, &e);
+    if (m < 7.07106781186547524401e-01) { m = m * 2.0; e = e - 1; }
+    s = (m - 1.0) / (m + 1.0);
+    s2 = s * s;
+    t = 1.0 / 23.0;
+    for (i = 21; i >= 1; i -= 2) t = t * s2 + 1.0 / (double)i;
+    de = (double)e;
+    return de * LH_LN2_HI + (de * LH_LN2_LO + 2.0 * s * t);
+}
+
+double lh_cbrt(double x)
+{
+    int e, q, it, neg;
+    double m, y;
+    if (x == 0.0 || x != x || x == HUGE_VAL || x == -HUGE_VAL) return x;
+    neg = x < 0.0;
+    m = frexp(neg ? -x : x, &e);
+    q = (e >= 0) ? e / 3 : -((2 - e) / 3);          /* floor(e / 3) */
+    m = ldexp(m, e - 3 * q) * 0.125;                /* [1/16, 1/2) */
+    q = q + 1;
+    y = 0.34016 + 0.90710 * m;
+    for (it = 0; it < 6; it++) y = y - (y * y * y - m) / (3.0 * y * y);
+    y = ldexp(y, q);
+    return neg ? -y : y;
+}
+
+static inline double lh_pow43(double h) { return h * lh_cbrt(h); }
+static inline double lh_powpos(double x, double y) { return lh_exp(y * lh_log(x)); }
 static inline double sgn(double x) { return (fabs(x) < LH_EPS) ? 0.0 : ((x > 0) ? 1.0 : -1.0); }
 
 /* U(h,q) = sqrt2*h*q / sqrt(h^4 + max(h,eps_h)^4)   (Arista_kernel.cu:378-379) */
@@ -71,7 +150,7 @@ double lh_friction(const lh_params *p, double h1ij, double u1ij, double h2ij, do
     {
         double r = p->r, eps = p->eps_h, g = p->g;
         double x = 10.0 * h1ij / eps;
-        double rr = 1.0 - r * (1.0 - exp(-(x * x)));
+        double rr = 1.0 - r * (1.0 - lh_exp(-(x * x)));
         double gp = g * rr;
         double fr1 = LH_SQRT2 * (u1ij * u1ij) * h1ij / (gp * sqrt(p4(h1ij) + p4(fmax(h1ij, eps))));
         double fr2 = LH_SQRT2 * (u2ij * u2ij) * h2ij / (gp * sqrt(p4(h2ij) + p4(fmax(h2ij, eps))));
@@ -86,7 +165,7 @@ double lh_friction(const lh_params *p, double h1ij, double u1ij, double h2ij, do
         else if (fabs(fr) < LH_EPS)
             muf = t3 + (t4 - t3) / (1.0 + h2ij / L2);
         else
-            muf = mustart + pow(fr / beta, chi) * (mustop - mustart);
+            muf = mustart + lh_powpos(fr / beta, chi) * (mustop - mustart);
         return muf * p->L / p->H;
     }
 }
@@ -233,7 +312,7 @@ int lh_edge(const lh_params *p, const double W0[6], const double W1[6], double H
         tp[3] = g * h2ij * ((1 - r) * deta2 + r * deta1);
         muc = lh_friction(p, h1ij, u1ij, h2ij, u2ij);
         x = 10.0 * h1ij / eps;
-        fsc = 1.0 - r * (1.0 - exp(-(x * x)));
+        fsc = 1.0 - r * (1.0 - lh_exp(-(x * x)));
         sc = fsc * muc * g * h2ij;
         coul = (fabs(h2ij * u2ij) < peso * sc * dt) ? 1 : 0;
         tp2[0] = tp[0]; tp2[1] = tp[1]; tp2[2] = tp[2];
@@ -372,10 +451,10 @@ void lh_filtro(const lh_params *p, double n1[3], double n2[3], double dt)
     if (n1[0] < 0.0) n1[0] = 0.0;
     if (n2[0] < 0.0) n2[0] = 0.0;
     aux0 = 1.0 / (p4(n1[0] / eps) + LH_EPS);
-    aux = exp(-dt * aux0);
+    aux = lh_exp(-dt * aux0);
     n1[1] *= aux; n1[2] *= aux;
     aux0 = 1.0 / (p4(n2[0] / eps) + LH_EPS);
-    aux = exp(-dt * aux0);
+    aux = lh_exp(-dt * aux0);
     n2[1] *= aux; n2[2] *= aux;
     if (p->vmax1 > 0.0) {
         double h = n1[0], hm = sqrt(p4(h) + p4(fmax(h, eps)));
@@ -415,7 +494,7 @@ void lh_dis_implicita(const lh_params *p, const double o1[3], const double o2[3]
     if ((h1 > 0) && (h2 > 0)) {
         double c1 = dt * LH_SQRT2 * h2 * hmod / hmodm * mfc * du;
         double c2 = dt * r * LH_SQRT2 * h1 * hmod / hmodm * mfc * du;
-        double c3 = dt * g * mfs * mfs * u2 / (pow(h2, 4.0 / 3.0) + LH_EPS);
+        double c3 = dt * g * mfs * mfs * u2 / (lh_pow43(h2) + LH_EPS);
         double det = 1.0 / ((1.0 + c1) * (1.0 + c2 + c3) - c1 * c2);
         n1[1] = h1 * (u1x * (1.0 + c2 + c3) + c1 * u2x) * det;
         n2[1] = h2 * (u2x * (1.0 + c1) + c2 * u1x) * det;
@@ -425,14 +504,14 @@ void lh_dis_implicita(const lh_params *p, const double o1[3], const double o2[3]
     if ((h1 > 0) && (h2 < eps)) {
         double c1;
         u1x = LH_SQRT2 * n1[1] * h1 / h1m; u1y = LH_SQRT2 * n1[2] * h1 / h1m;
-        c1 = dt * g * mf0 * mf0 * u1 / (pow(h1, 4.0 / 3.0) + LH_EPS);
+        c1 = dt * g * mf0 * mf0 * u1 / (lh_pow43(h1) + LH_EPS);
         n1[1] = h1 * u1x / (1.0 + c1);
         n1[2] = h1 * u1y / (1.0 + c1);
     }
     if ((h2 > 0) && (h1 < eps)) {
         double c1;
         u2x = LH_SQRT2 * n2[1] * h2 / h2m; u2y = LH_SQRT2 * n2[2] * h2 / h2m;
-        c1 = dt * g * mfs * mfs * u2 / (pow(h2, 4.0 / 3.0) + LH_EPS);
+        c1 = dt * g * mfs * mfs * u2 / (lh_pow43(h2) + LH_EPS);
         n2[1] = h2 * u2x / (1.0 + c1);
         n2[2] = h2 * u2y / (1.0 + c1);
     }
@@ -447,7 +526,7 @@ void lh_coulomb(const lh_params *p, double n1[3], double n2[3], double dt)
         u1 = LH_SQRT2 * sqrt(n1[1] * n1[1] + n1[2] * n1[2]) * n1[0] / sqrt(p4(n1[0]) + p4(fmax(n1[0], eps)));
         u2 = LH_SQRT2 * sqrt(n2[1] * n2[1] + n2[2] * n2[2]) * n2[0] / sqrt(p4(n2[0]) + p4(fmax(n2[0], eps)));
     }
-    fsc = 1.0 - r * (1.0 - exp(-(x * x)));
+    fsc = 1.0 - r * (1.0 - lh_exp(-(x * x)));
     muc = lh_friction(p, n1[0], u1, n2[0], u2);
     sc = fsc * muc * g * n2[0] * dt * 1.0;
     normq = sqrt(n2[1] * n2[1] + n2[2] * n2[2]);

@@ -54,3 +54,15 @@ def test_batch_rejects_invalid_scenarios_before_touching_a_device():
         solver.Batch([q])
     assert e.value.code == 3
     assert len(solver.batch_tile_shapes()) >= 2
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_create_rejects_unknown_friction_law_before_touching_a_device(precision):
+    """friction_law outside {0, 1} is an argument error in both precisions
+    (no silent Pouliquen for other nonzero values)."""
+    from landslidehysea_amd import solver
+    for law in (2, -1):
+        p = solver.Params(nx=8, ny=8, dx=1.0, dy=1.0, precision=precision, friction_law=law)
+        with pytest.raises(solver.LhyseaError) as e:
+            solver.Simulation(p)
+        assert e.value.code == 3 and "friction_law" in str(e.value)

@@ -106,6 +106,41 @@ def test_config1_400x300_matches_oracle(oracle, angle):
         assert abs(m1 - m0) <= 1e-12 * m0
 
 
+def test_config1_literal_2000_steps_six_gauges(oracle):
+    """The 400x300 plain-FP64 workload as stated: 10 deg, 2000 steps, 6 seeded
+    gauges, eta max tracked; state, gauge series, eta max, dt history and time
+    against the oracle."""
+    from landslidehysea_amd.scenario import gauge_cells
+    steps = 2000
+    sc, W, H, prm = case(400, 300, angle=10.0)
+    gauges = gauge_cells(sc, 6, seed=7)
+    say(f"config 1 literal: 400x300, {steps} steps, gauges {gauges.tolist()}")
+    po = oracle_params(oracle, prm)
+    Wo = W.copy()
+    em_o = np.stack([(W[0] + W[3] - H).ravel(), np.zeros(W[0].size)], axis=1).copy()
+    res = oracle.run(po, Wo, H, steps, eta_max=em_o, gauges=gauges)
+    with L.Simulation(prm) as sim:
+        sim.upload(W, H)
+        sim.track_eta_max(True)
+        sim.set_gauges(gauges, steps)
+        sim.step(steps)
+        Wg = sim.download()
+        ev, et = sim.eta_max()
+        gs = sim.gauges()
+        t, dtn, n = sim.time()
+        hist = sim.dt_history(steps)
+    errs, bad = compare(Wg, Wo, H)
+    rest = float((W[0] + W[3] - H)[W[0] > 1.0].mean())
+    g_err = rel_l2(gs - rest, res["gauges"] - rest)
+    say("rel L2:", errs, "gauge signal rel L2", g_err, "eta max", rel_l2(ev.ravel(), em_o[:, 0]))
+    assert not bad, bad
+    assert gs.shape == (steps, 6) and g_err <= TOL
+    assert rel_l2(ev.ravel(), em_o[:, 0]) <= TOL
+    assert np.allclose(et.ravel(), em_o[:, 1], rtol=1e-12, atol=0)
+    assert n == steps and abs(t - res["t"]) <= 1e-12 * res["t"]
+    assert np.allclose(hist, res["dt_hist"], rtol=1e-12, atol=0)
+
+
 def test_eta_max_and_gauges_match_oracle(oracle):
     sc, W, H, prm = case(96, 64, angle=2.0, semi_axes=(20.0, 14.0))
     from landslidehysea_amd.scenario import gauge_cells

@@ -227,6 +227,85 @@ def test_vmax_cap(oracle):
     assert math.hypot(n1[1], n1[2]) / 4.0 == pytest.approx(2.0, rel=1e-9)
 
 
+@pytest.mark.parametrize("h", [4.0, 5e-3, 2e-3, 4e-4])
+def test_vmax_cap_sets_discharge_to_vmax_times_depth(oracle, h):
+    """The cap rescales the desingularised velocity to vmax and stores q = u h
+    (src/GPU/Volumen_kernel.cu:150-179), so |q| = vmax h whatever h; below
+    eps_h the velocity of the capped state, |q| s(h) = vmax h s(h), is then
+    under vmax.  Scaling q itself by vmax/u (dropping the h s(h) factor)
+    gives |q| = vmax / s(h), three times vmax h at h = eps_h / 2."""
+    vmax1, vmax2, eps = 1.5, 0.8, 5e-3
+    p = prm(oracle, vmax1=vmax1, vmax2=vmax2)
+    s = math.sqrt(2) * h / math.sqrt(h ** 4 + max(h, eps) ** 4)     # U(h, q) = q s(h)
+    q1, q2 = np.array([30.0, -40.0]) / s, np.array([-6.0, 8.0]) / s  # speeds 50 and 10
+    assert 10.0 > max(vmax1, vmax2)
+    dt = 1e-12                                    # no thin-layer damping
+    n1, n2 = oracle.filtro(p, [h, *q1], [h, *q2], dt)
+    assert math.hypot(n1[1], n1[2]) == pytest.approx(vmax1 * h, rel=1e-12)
+    assert math.hypot(n2[1], n2[2]) == pytest.approx(vmax2 * h, rel=1e-12)
+    assert n1[1] / n1[2] == pytest.approx(q1[0] / q1[1], rel=1e-12)    # direction kept
+    assert n2[1] / n2[2] == pytest.approx(q2[0] / q2[1], rel=1e-12)
+
+
+def test_dis_implicita_interlayer_then_water_manning(oracle):
+    """h1 >= eps_h > h2 > 0 takes the coupled case (i) and then the water-bottom
+    case (ii) (src/GPU/Volumen_kernel.cu:64-82, sequential ifs): (ii) divides
+    (i)'s layer-1 output by 1 + c, c = dt g mf0^2 |u1_old| / (h1^(4/3) + EPS),
+    and leaves layer 2 as (i) left it.  mf0 = 0 isolates (i) (its (ii) factor
+    is h1 s(h1) = 1), whose output solves the 2x2 friction system."""
+    eps, r, dt, mfc, mfs, mf0 = 5e-3, 0.45, 0.3, 0.08, 0.05, 0.03
+    o1, o2 = [12.0, 3.0, -1.0], [2e-3, 1e-4, 5e-5]
+    n1, n2 = [11.5, 2.5, -0.8], [3e-3, 2e-4, -1e-4]
+    pa = prm(oracle, mfc=mfc, mfs=mfs, mf0=0.0, r=r)
+    pb = prm(oracle, mfc=mfc, mfs=mfs, mf0=mf0, r=r)
+    a1, a2 = oracle.dis_implicita(pa, o1, o2, n1, n2, dt)
+    b1, b2 = oracle.dis_implicita(pb, o1, o2, n1, n2, dt)
+
+    def U(h, q):                                  # desingularised velocity
+        return math.sqrt(2) * h * np.asarray(q) / math.sqrt(h ** 4 + max(h, eps) ** 4)
+
+    h1, h2 = n1[0], n2[0]
+    uo1, uo2 = U(o1[0], o1[1:]), U(o2[0], o2[1:])
+    du = np.linalg.norm(uo1 - uo2)
+    hmod = h2 + r * h1
+    phi = math.sqrt(2) * hmod / math.sqrt(hmod ** 4 + max(hmod, eps * (1 + r)) ** 4)
+    c1, c2 = dt * h2 * phi * mfc * du, dt * r * h1 * phi * mfc * du
+    c3 = dt * G * mfs ** 2 * np.linalg.norm(uo2) / (h2 ** (4 / 3) + EPS)
+    v1, v2 = np.array(a1[1:]) / h1, np.array(a2[1:]) / h2
+    u1, u2 = U(h1, n1[1:]), U(h2, n2[1:])
+    assert np.allclose((1 + c1) * v1 - c1 * v2, u1, rtol=1e-12)
+    assert np.allclose(-c2 * v1 + (1 + c2 + c3) * v2, u2, rtol=1e-12)
+    c = dt * G * mf0 ** 2 * np.linalg.norm(uo1) / (h1 ** (4 / 3) + EPS)
+    assert np.allclose(b1[1:], np.array(a1[1:]) / (1 + c), rtol=1e-13)
+    assert np.array_equal(b2, a2)
+
+
+def test_dis_implicita_sediment_only_manning(oracle):
+    """h1 = 0 < h2: only the sediment-bottom case (iii) applies
+    (src/GPU/Volumen_kernel.cu:83-89): q2 / (1 + c), c built from mfs (not
+    mf0) and the old layer-2 speed; layer 1 untouched."""
+    dt, mfs, mf0 = 0.4, 0.05, 0.02
+    p = prm(oracle, mfc=0.08, mfs=mfs, mf0=mf0, r=0.5)
+    o2, n2 = [4.0, 0.4, 0.2], [4.2, 0.7, 0.1]
+    out1, out2 = oracle.dis_implicita(p, [0.0, 0.0, 0.0], o2, [0.0, 0.0, 0.0], n2, dt)
+    c = dt * G * mfs ** 2 * math.hypot(0.4 / 4.0, 0.2 / 4.0) / (4.2 ** (4 / 3) + EPS)
+    assert np.allclose(out2[1:], np.array(n2[1:]) / (1 + c), rtol=1e-13)
+    assert list(out1) == [0.0, 0.0, 0.0]
+
+
+def test_dis_implicita_interlayer_then_sediment_manning(oracle):
+    """0 < h1 < eps_h <= h2 takes the coupled case (i) and then (iii).  With
+    mfc = 0 the coupled case reduces to v2 = u2 / (1 + c3), and (iii) applies
+    the same sediment-bottom factor again: q2 / (1 + c3)^2."""
+    dt, mfs = 0.4, 0.05
+    p = prm(oracle, mfc=0.0, mfs=mfs, mf0=0.02, r=0.5)
+    o1, o2 = [1e-3, 1e-4, 0.0], [4.0, 0.4, 0.2]
+    n1, n2 = [1e-3, 1e-4, 2e-5], [4.2, 0.7, 0.1]
+    _, out2 = oracle.dis_implicita(p, o1, o2, n1, n2, dt)
+    c3 = dt * G * mfs ** 2 * math.hypot(0.4 / 4.0, 0.2 / 4.0) / (4.2 ** (4 / 3) + EPS)
+    assert np.allclose(out2[1:], np.array(n2[1:]) / (1 + c3) ** 2, rtol=1e-13)
+
+
 def test_cell_update_dt_contribution(oracle):
     p = prm(oracle, cfl=0.5)
     _, d = oracle.cell_update(p, np.array([5.0, 0, 0, 0, 0, 0]), np.array([0, 0, 0, 0, 0, 0, 80.0]), 0.1)
@@ -257,6 +336,31 @@ def test_initial_dt_uniform_state(oracle):
     # see the reflected ghost (u_ij = 0 there) and give smaller sums
     uij = (3.0 * 0.2 + 3.0 * 0.2) / (6.0 + EPS)
     A = 2 * dy * (uij + c) + 2 * dx * c
+    assert dt == pytest.approx(2 * 0.5 * dx * dy / A, rel=1e-13)
+
+
+@pytest.mark.parametrize("ufrac", [-0.05, -0.7])
+def test_initial_dt_signed_speed(oracle, ufrac):
+    """The initial edge speed is max(D_min, D_max, |u1ij|, |u2ij|) with the
+    signed eigenvalues (src/GPU/Arista_kernel.cu:1107-1110): for a single layer
+    flowing in -x, an interior x-edge gives max(u + c, |u|), not c + |u|.
+    u = -0.05 c: u + c wins; u = -0.7 c: |u| wins.  Interior x-edges are then
+    slower than c, so the largest sum belongs to a cell of the first or last
+    column: one wall edge (reflected ghost, u_ij = 0: speed c), one interior
+    x-edge and two y-edges at c."""
+    nx, ny, dx, dy, h = 6, 5, 30.0, 20.0, 9.0
+    c = math.sqrt(G * h)
+    u = ufrac * c
+    p = oracle.make_params(nx, ny, dx, dy, cfl=0.5)
+    W = np.zeros((6, ny, nx))
+    W[0] = h
+    W[1] = u * h
+    H = np.full((ny, nx), h)
+    dt = oracle.initial_dt(p, W, H)
+    uij = (3.0 * u + 3.0 * u) / (6.0 + EPS)
+    lam = max(uij + c, abs(uij))
+    A = dy * (c + lam) + 2 * dx * c
+    assert lam < c
     assert dt == pytest.approx(2 * 0.5 * dx * dy / A, rel=1e-13)
 
 
