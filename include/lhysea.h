@@ -211,6 +211,28 @@ int lhysea_batch_time_steps(lhysea_batch *b, int32_t nsteps, double *total_ms);
  * best may be NULL. */
 int lhysea_batch_autotune(lhysea_batch *b, int32_t nsteps, double *ms_per_tile, int32_t *best);
 
+/* Steps without device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The caller owns
+ * every buffer, all in device memory: H (nx*ny doubles, Hmin-shifted depth),
+ * W (the state: LHYSEA_FP64 6 planes of nx*ny doubles as in lhysea_upload;
+ * LHYSEA_FP32C 12 planes of nx*ny floats, the 6 FP32 values then their 6
+ * compensations) and W_work (a second buffer of W's layout).  nsteps fused
+ * steps run on `stream` (a hipStream_t; NULL: the default stream) from time
+ * *t with first dt *dt (<= 0: the initial-dt pass), as lhysea_step does on a
+ * handle (same kernels, same results); on return *t and *dt are the time and
+ * the next dt, and *result_in_work says where the final state is (0: W, 1:
+ * W_work; the other buffer holds the state one step earlier).  No eta max,
+ * gauges or dt history.  The library's own scratch (control block and tile
+ * class tables) is a fixed per-GPU array: grids of at most 65536 class tiles
+ * (FP64 tiles of 30 x 14 cells, FP32C 62 x 14; every grid up to 4096 x 4096
+ * fits), larger ones are refused with LHYSEA_ERR_ARGUMENT.  Calls that
+ * overlap on one GPU share that array and race; run them one after another.
+ * The call waits for its stream once after the tile classification and once
+ * at the end (it reads the control block back). */
+int lhysea_step_device(const lhysea_params *p, const double *H, void *W, void *W_work, int32_t nsteps,
+                       double *t, double *dt, int32_t *result_in_work, void *stream);
+
 /* One-shot driver with shallowWater's contract: runs while t < t_end, writes
  * no files itself (output is the caller's, see L-HySEA.exe), returns the loop
  * wall time in *elapsed_s.  W is updated in place with the final state; eta

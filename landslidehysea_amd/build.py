@@ -9,7 +9,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblhysea_hip.so")
-SOURCES = ["lhysea_capi.hip", "lhysea_batch.hip", "lhysea_fp64.hip", "lhysea_fp32c.hip", "lhysea_adjoint.hip"]
+SOURCES = ["lhysea_capi.hip", "lhysea_batch.hip", "lhysea_fp64.hip", "lhysea_fp32c.hip", "lhysea_adjoint.hip",
+           "lhysea_noalloc.hip"]
 ARCH = os.environ.get("LHYSEA_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions stay off in every HIP translation unit.

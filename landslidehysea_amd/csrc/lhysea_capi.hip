@@ -38,6 +38,20 @@ int hip_fail(hipError_t e, const char* where)
 
 void lhysea::set_last_error(const std::string& msg) { g_err = msg; }
 
+int lhysea::check_params(const lhysea_params& p)
+{
+    if (p.nx < 1 || p.ny < 1) return fail(LHYSEA_ERR_ARGUMENT, "nx and ny must be positive");
+    if (p.precision != LHYSEA_FP64 && p.precision != LHYSEA_FP32C)
+        return fail(LHYSEA_ERR_ARGUMENT, "unknown precision");
+    if (!(p.dx > 0) || !(p.dy > 0) || !(p.eps_h > 0) || !(p.H > 0) || !(p.peso > 0))
+        return fail(LHYSEA_ERR_ARGUMENT, "dx, dy, eps_h, H and peso must be positive");
+    if (p.friction_law != 0 && p.friction_law != 1)
+        return fail(LHYSEA_ERR_ARGUMENT, "friction_law must be 0 (Coulomb) or 1 (Pouliquen)");
+    if (p.precision == LHYSEA_FP32C && (p.peso != 1.0 || p.beta != 1.0))
+        return fail(LHYSEA_ERR_ARGUMENT, "the FP32C path implements peso = beta = 1");
+    return LHYSEA_OK;
+}
+
 struct lhysea_adj_work;
 
 struct lhysea_sim {
@@ -223,15 +237,7 @@ int lhysea_create(const lhysea_params* p, lhysea_sim** out)
 {
     if (!p || !out) return fail(LHYSEA_ERR_ARGUMENT, "null argument");
     *out = nullptr;
-    if (p->nx < 1 || p->ny < 1) return fail(LHYSEA_ERR_ARGUMENT, "nx and ny must be positive");
-    if (p->precision != LHYSEA_FP64 && p->precision != LHYSEA_FP32C)
-        return fail(LHYSEA_ERR_ARGUMENT, "unknown precision");
-    if (!(p->dx > 0) || !(p->dy > 0) || !(p->eps_h > 0) || !(p->H > 0) || !(p->peso > 0))
-        return fail(LHYSEA_ERR_ARGUMENT, "dx, dy, eps_h, H and peso must be positive");
-    if (p->friction_law != 0 && p->friction_law != 1)
-        return fail(LHYSEA_ERR_ARGUMENT, "friction_law must be 0 (Coulomb) or 1 (Pouliquen)");
-    if (p->precision == LHYSEA_FP32C && (p->peso != 1.0 || p->beta != 1.0))
-        return fail(LHYSEA_ERR_ARGUMENT, "the FP32C path implements peso = beta = 1");
+    if (const int rc = check_params(*p)) return rc;
     if (lhysea_device_probe() != 0) return fail(LHYSEA_ERR_NO_DEVICE, "no usable HIP device");
     lhysea_sim* s = new (std::nothrow) lhysea_sim;
     if (!s) return fail(LHYSEA_ERR_HOST_MEMORY, "host allocation failed");

@@ -1170,8 +1170,7 @@ hipError_t f32c_classify(const Consts& k, const float* W, const StepArgs& a, hip
 {
     const int nt = f32c_tiles(k);
     const TileCls tc{a.tcls, a.tlist, (k.nx + f32::FTX - 1) / f32::FTX, (k.ny + f32::FTY - 1) / f32::FTY};
-    hipError_t e = hipMemsetAsync(a.tcls, 0xFF, 2 * (size_t)nt * sizeof(int), s);   // no tile general (-1)
-    if (e == hipSuccess) e = hipMemsetAsync(a.ctl->gcount, 0, sizeof(a.ctl->gcount), s);
+    hipError_t e = clear_classes(a.tcls, 2 * nt, a.ctl, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((f32::k_classify<f32::FTX, f32::FTY>), dim3((unsigned)nt), dim3(256), 0, s, k, W, a, tc);
     return hipGetLastError();
@@ -1228,8 +1227,7 @@ const BatchTile kBatchTiles[kNumBatchTiles] = {{62, 14, 512}, {30, 14, 256}, {62
 hipError_t f32c_batch_classify(const BatchScen& d, const BatchScen* sd_dev, int sid, hipStream_t s)
 {
     const int nct = d.ctiles_x * d.ctiles_y;
-    hipError_t e = hipMemsetAsync(d.cls, 0xFF, 2 * (size_t)nct * sizeof(int), s);   // no tile general (-1)
-    if (e == hipSuccess) e = hipMemsetAsync(d.ctl->gcount, 0, sizeof(d.ctl->gcount), s);
+    hipError_t e = clear_classes(d.cls, 2 * nct, d.ctl, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(f32::k_batch_classify, dim3((unsigned)nct), dim3(256), 0, s, sd_dev, sid);
     return hipGetLastError();

@@ -818,8 +818,7 @@ hipError_t f64_classify(const Consts& k, const double* W, const StepArgs& a, hip
 {
     const int nt = f64_tiles(k);
     const TileCls tc{a.tcls, a.tlist, (k.nx + f64::FTX - 1) / f64::FTX, (k.ny + f64::FTY - 1) / f64::FTY};
-    hipError_t e = hipMemsetAsync(a.tcls, 0xFF, 2 * (size_t)nt * sizeof(int), s);   // no tile general (-1)
-    if (e == hipSuccess) e = hipMemsetAsync(a.ctl->gcount, 0, sizeof(a.ctl->gcount), s);
+    hipError_t e = clear_classes(a.tcls, 2 * nt, a.ctl, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(f64::k_classify, dim3((unsigned)nt), dim3(256), 0, s, k, W, a, tc);
     return hipGetLastError();

@@ -14,6 +14,8 @@ namespace lhysea {
 // Shared by the C-ABI files (lhysea_capi.hip)
 Consts make_consts(const lhysea_params& p);
 void set_last_error(const std::string& msg);
+// argument checks of lhysea_create (0, or an LHYSEA_ERR_ code with the message set)
+int check_params(const lhysea_params& p);
 
 enum Precision { PREC_FP64 = 0, PREC_FP32C = 1 };
 enum KernelMode { MODE_FOURPASS = 0, MODE_FUSED = 1 };

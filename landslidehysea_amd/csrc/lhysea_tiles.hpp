@@ -22,6 +22,25 @@ struct TileCls {
     int tiles_x, tiles_y;
 };
 
+// Clears a class table before a classification: n entries -1 (no tile
+// general at any step) and both general counts 0.  A kernel rather than a
+// memset, so that a launch sequence that contains it can be captured into a
+// graph and replayed.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_clear_classes(int* __restrict__ cls, int n, Ctl* ctl)
+{
+    for (int i = (int)(blockIdx.x * NT + threadIdx.x); i < n; i += (int)(gridDim.x * NT)) cls[i] = -1;
+    if (blockIdx.x == 0 && threadIdx.x < 2) ctl->gcount[threadIdx.x] = 0u;
+}
+
+inline hipError_t clear_classes(int* cls, int n, Ctl* ctl, hipStream_t s)
+{
+    const int b = (n + 255) / 256;
+    hipLaunchKernelGGL(k_clear_classes<256>, dim3((unsigned)(b < 1 ? 1 : (b > 1024 ? 1024 : b))), dim3(256), 0, s,
+                       cls, n, ctl);
+    return hipGetLastError();
+}
+
 // Class tiles whose tile + halo box holds the cell at (lx, ly) of a CTX x CTY
 // class tile, as bits (dy + 1) * 3 + (dx + 1) for the neighbour at (dx, dy):
 // the tile itself and the neighbours across the edges and corners the cell

@@ -127,6 +127,8 @@ SIGNATURES = {
     "lhysea_adjoint": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, C.c_int32, _dp, _dp, _dp, _dp, _dp,
                                  _dp]),
     "lhysea_shallow_water": (C.c_int, [C.POINTER(CParams), _dp, _dp, C.c_double, _dp, _dp, _dp, _ip]),
+    "lhysea_step_device": (C.c_int, [C.POINTER(CParams), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, _dp, _dp,
+                                     _ip, C.c_void_p]),
     "lhysea_batch_create": (C.c_int, [C.POINTER(CParams), C.c_int32, C.POINTER(C.c_void_p)]),
     "lhysea_batch_destroy": (C.c_int, [C.c_void_p]),
     "lhysea_batch_size": (C.c_int, [C.c_void_p, _ip]),
@@ -474,3 +476,17 @@ def shallow_water(params: Params, W: np.ndarray, H: np.ndarray, t_end: float):
     _check(lib.lhysea_shallow_water(C.byref(cp), _dptr(W), _dptr(H), t_end, _dptr(em), _dptr(et),
                                     C.byref(el), C.byref(n)))
     return W, em.reshape(params.ny, params.nx), et.reshape(params.ny, params.nx), el.value, n.value
+
+
+def step_device(params: Params, H_dev: int, W_dev: int, W_work_dev: int, nsteps: int, t: float = 0.0,
+                dt: float = 0.0, stream: int | None = None):
+    """Steps of a state in caller-owned device memory without allocation
+    (lhysea_step_device): device addresses of the depth, the state and a
+    second state buffer of its layout.  Returns (t, dt_next, result_in_work)."""
+    lib = load_library()
+    cp = params.to_c()
+    tt, dd, w = C.c_double(t), C.c_double(dt), C.c_int32()
+    _check(lib.lhysea_step_device(C.byref(cp), C.c_void_p(H_dev), C.c_void_p(W_dev), C.c_void_p(W_work_dev),
+                                  int(nsteps), C.byref(tt), C.byref(dd), C.byref(w),
+                                  C.c_void_p(stream) if stream else None))
+    return tt.value, dd.value, w.value

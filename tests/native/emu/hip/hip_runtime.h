@@ -32,6 +32,7 @@ using std::max;
 #define __shared__ static
 #define __restrict__
 #define HIP_SYMBOL(x) (&x)
+#define __align__(n) __attribute__((aligned(n)))
 
 struct dim3 { unsigned x, y, z; dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {} };
 struct EmuIdx { unsigned x, y, z; };
@@ -118,6 +119,7 @@ inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) { p->major = 9
 inline hipError_t hipGetDevice(int* d) { *d = 0; return 0; }
 template <class T> inline hipError_t hipMalloc(T** p, size_t n) { *p = (T*)calloc(n ? n : 1, 1); return *p ? 0 : 2; }
 inline hipError_t hipFree(void* p) { free(p); return 0; }
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_cast<void*>(sym); return 0; }
 inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { std::memmove(d, s, n); return 0; }
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) { std::memmove(d, s, n); return 0; }
 inline hipError_t hipMemset(void* d, int v, size_t n) { std::memset(d, v, n); return 0; }

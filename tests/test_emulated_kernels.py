@@ -47,7 +47,8 @@ import pytest
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(ROOT, "landslidehysea_amd", "csrc")
-SOURCES = ["lhysea_capi.hip", "lhysea_batch.hip", "lhysea_fp64.hip", "lhysea_fp32c.hip", "lhysea_adjoint.hip"]
+SOURCES = ["lhysea_capi.hip", "lhysea_batch.hip", "lhysea_fp64.hip", "lhysea_fp32c.hip", "lhysea_adjoint.hip",
+           "lhysea_noalloc.hip"]
 
 
 def build_emulated(out: str):
