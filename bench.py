@@ -1,23 +1,25 @@
 """Benchmark of the LandSlide-HySEA solver step on one MI355X.
 
-Workload: the 2048x2048 precision-comparison case (plane shelf + 8 seeded
-Gaussian bumps, 4 seeded slides of 20-80 m, water at rest, walls, CFL 0.5,
-r 0.5, Coulomb 10 deg), forward steps of the fused step.  A "step" is one
-solver time step over the whole grid: the four edge classes, the cell update
-and the on-device dt reduction; in FP32C its launches are the sediment-free
-tiles' kernel and the general tiles' kernel, which also closes the step
-(tile_classes reports how many class tiles took the general one).
+Headline workload: the large single-GPU case, 4096x4096 (plane shelf + 8
+seeded Gaussian bumps, 3 seeded slides of 20-80 m, water at rest, walls, CFL
+0.5, r 0.5, Coulomb 10 deg), forward steps of the fused step.  A "step" is
+one solver time step over the whole grid: the four edge classes, the cell
+update and the on-device dt reduction; in FP32C its launches are the
+sediment-free tiles' kernel and the general tiles' kernel, which also closes
+the step (tile_classes reports how many class tiles took the general one).
 
 Prints one JSON line.  value = Mcell-updates/s of the headline path (FP32
 with compensated state when available, else FP64) with the state resident in
 HBM; the FP64 path is timed beside it.  roofline.achieved uses algorithmic
 bytes per cell-update (104 B: read 6 state fields + depth, write 6 fields)
-over the step kernel's average duration measured with HIP events on the
-library's stream.  cpu_baseline times the FP64 oracle on the host cores.
-precision_2048 reports FP32C against FP64 on the same construction (gauges,
-eta, h_s, maximum runup) at the workload's 10 deg and with the slides flowing
-at 2 deg, beside the spread that FP32-sized rounding of the input alone causes
-in the FP64 solver.
+over the step kernels' average duration measured with HIP events on the
+library's stream; roofline.traffic is the HBM bytes per step that rocprofv3's
+FETCH_SIZE / WRITE_SIZE counters measured for the same workload.  Extra keys:
+the 2048x2048 precision-comparison workload's forward steps (c2_2048), the
+adjoint workloads (1024x1024 and 4096x4096), the 64-scenario ensemble (a
+short timed run and the workload's full 5000 steps), the FP32C-vs-FP64
+precision report at 2048x2048, and cpu_baseline (the FP64 oracle on the host
+cores).
 """
 from __future__ import annotations
 
@@ -32,6 +34,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baseline's OpenMP threads stay on their cores (read once, when the
+# OpenMP runtime starts)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 from landslidehysea_amd import scenario as S          # noqa: E402
 from landslidehysea_amd import solver as L            # noqa: E402
@@ -40,27 +46,32 @@ from landslidehysea_amd import solver as L            # noqa: E402
 # 6x8 B; FP32c reads 6 values + 6 compensations (12x4 B) + 8 B depth and writes 12x4 B
 BYTES_PER_CELL = 104
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per FP32C step at 2048x2048 (the sediment-free and the general
-# tiles' launches together), measured with rocprofv3 FETCH_SIZE (x2.0) and
-# WRITE_SIZE (x1.0), the scale factors of a plain 256 MiB copy
+# HBM bytes per FP32C step at 2048x2048 and 4096x4096 (the sediment-free and
+# the general tiles' launches together), measured with rocprofv3 FETCH_SIZE
+# (x2.0) and WRITE_SIZE (x1.0), the scale factors of a plain 256 MiB copy
 # (profiles/fused_step_fp32c_2048x2048_counters.txt,
+# profiles/fused_step_fp32c_4096x4096_counters.txt,
 # profiles/copy_256MiB_counter_calibration.txt); below the algorithmic bytes
 # because sediment-free tiles neither read their zero layer-2 planes nor write
 # them again where the output buffer already holds them
-TRAFFIC_BYTES_FP32C = 256.2e6
-NX = NY = 2048
+TRAFFIC_BYTES_FP32C = {2048: 256.2e6, 4096: 976.5e6}
+# workloads: grid side -> (seed, slides)
+WORKLOADS = {2048: (2048, 4), 4096: (4096, 3)}
+NX = NY = 2048      # the precision-comparison workload
 SEED = 2048
+HEAD_N = 4096       # the headline workload
 
 
 def make_case(nx=NX, ny=NY, precision=L.FP64):
-    sc = S.make_scenario(nx, ny, seed=SEED, n_slides=4, thickness=(20.0, 80.0))
+    seed, slides = WORKLOADS.get(nx, (SEED, 4))
+    sc = S.make_scenario(nx, ny, seed=seed, n_slides=slides, thickness=(20.0, 80.0))
     H, _ = S.hmin_shift(sc.H)
     prm = L.Params(nx=nx, ny=ny, dx=sc.dx, dy=sc.dy, angle_deg=(10.0,), precision=precision)
     return sc, np.ascontiguousarray(sc.state), np.ascontiguousarray(H), prm
 
 
-def time_gpu(precision, steps, warmup):
-    sc, W, H, prm = make_case(precision=precision)
+def time_gpu(precision, steps, warmup, n=HEAD_N):
+    sc, W, H, prm = make_case(n, n, precision=precision)
     with L.Simulation(prm) as sim:
         sim.upload(W, H)
         sim.initial_dt()
@@ -73,10 +84,11 @@ def time_gpu(precision, steps, warmup):
         wall = time.perf_counter() - t0
         _, kernel_ms = sim.time_steps(max(steps // 2, 5), per_kernel=True)
         classes = sim.tile_classes()
-        t, dtn, n = sim.time()
-    N = NX * NY
-    return {"total_ms": total_ms, "wall_s": wall, "kernel_ms": kernel_ms, "classes": classes,
-            "mcells": N * steps / (total_ms * 1e-3) / 1e6, "t": t, "steps_done": n}
+        t, dtn, done = sim.time()
+    N = n * n
+    return {"n": n, "total_ms": total_ms, "wall_s": wall, "kernel_ms": kernel_ms, "classes": classes,
+            "mcells": N * steps / (total_ms * 1e-3) / 1e6, "t": t, "steps_done": done,
+            "achieved_gbs": BYTES_PER_CELL * N / (kernel_ms * 1e-3) / 1e9}
 
 
 def time_adjoint(precision, steps=4000, snaps=64, n=1024, ngauges=16):
@@ -145,41 +157,28 @@ def precision_report(steps=1000, angle=2.0, ngauges=16):
             "max_runup_m": {"fp64": float(e64[wet].max()) - rest, "fp32c": float(e32[wet].max()) - rest}}
 
 
-def time_large(steps=100, adj_steps=200, snaps=100, n=4096):
-    """4096x4096 workload: three seeded slides of 20-80 m on the same shelf
-    construction; FP32C forward steps and the discrete adjoint of a bounded
-    prefix with `snaps` HBM snapshots (0.8 GB each)."""
-    sc = S.make_scenario(n, n, seed=n, n_slides=3, thickness=(20.0, 80.0))
-    H, _ = S.hmin_shift(sc.H)
-    W = np.ascontiguousarray(sc.state)
-    H = np.ascontiguousarray(H)
-    prm = L.Params(nx=n, ny=n, dx=sc.dx, dy=sc.dy, angle_deg=(10.0,), precision=L.FP32C)
-    with L.Simulation(prm) as sim:
-        sim.upload(W, H)
-        sim.initial_dt()
-        sim.step(10)
-        sim.sync()
-        total_ms, kernel_ms = sim.time_steps(steps, per_kernel=True)
+def time_large_adjoint(adj_steps=200, snaps=100, n=4096):
+    """4096x4096 workload: the discrete adjoint (FP32C) of a bounded prefix of
+    its 20000 steps with `snaps` HBM snapshots (0.8 GB each)."""
+    sc, W, H, prm = make_case(n, n, precision=L.FP32C)
     gauges = S.gauge_cells(sc, 16, seed=16)
     obs = np.zeros((adj_steps, len(gauges)))
     with L.Simulation(prm) as sim:
         sim.upload(W, H)
         res = sim.adjoint(adj_steps, snaps, gauges, obs)
     fwd, rev = res["forward_ms"], res["reverse_ms"]
-    N = n * n
-    return {"grid": f"{n}x{n}", "steps": steps, "mcells": round(N * steps / (total_ms * 1e-3) / 1e6, 2),
-            "kernel_ms": round(kernel_ms, 4),
-            "achieved_gbs": round(BYTES_PER_CELL * N / (kernel_ms * 1e-3) / 1e9, 1),
-            "adjoint": {"steps": adj_steps, "snapshots": snaps, "forward_ms": round(fwd, 2),
-                        "reverse_ms": round(rev, 2), "cost_vs_forward": round((fwd + rev) / fwd, 3),
-                        "re_advances": res["advances"]}}
+    return {"grid": f"{n}x{n}", "steps": adj_steps, "snapshots": snaps, "forward_ms": round(fwd, 2),
+            "reverse_ms": round(rev, 2), "cost_vs_forward": round((fwd + rev) / fwd, 3),
+            "re_advances": res["advances"]}
 
 
 def time_ensemble(steps=200, warmup=16, n=64, tune_steps=16):
     """Ensemble workload: 64 seeded scenarios (sides U{256..1536}, one slide of
     volume logU[1e6, 5e7] m^3, friction U[6, 16] deg, r U[0.45, 0.6]) stepped
     together in FP32C by the batched kernel, the step loop replayed from a
-    captured graph; the tile shape is autotuned first."""
+    captured graph; the tile shape is autotuned first (the autotuner restores
+    the state), then `warmup` untimed and `steps` timed steps (warmup 0 and
+    5000 steps: every scenario runs the workload's 5000 steps)."""
     specs, build = S.make_ensemble(n)
     params = []
     for spec in specs:
@@ -195,7 +194,8 @@ def time_ensemble(steps=200, warmup=16, n=64, tune_steps=16):
             del sc, H
         b.initial_dt()
         best, ms = b.autotune(tune_steps)
-        b.step(warmup)
+        if warmup:
+            b.step(warmup)
         b.sync()
         total_ms = b.time_steps(steps)
         t0 = [b.time(i) for i in (0, n - 1)]
@@ -208,28 +208,72 @@ def time_ensemble(steps=200, warmup=16, n=64, tune_steps=16):
             "steps_done_first_last": [t0[0][2], t0[1][2]]}
 
 
-def cpu_baseline(sample_steps=3, nx=1024, ny=1024):
-    """FP64 oracle (OpenMP) on the host: a bounded sample of the same construction."""
+def host_cpu():
+    """CPU model and physical core count of the host (/proc/cpuinfo)."""
+    model, cores = "unknown", set()
+    try:
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+    except OSError:
+        pass
+    return model, len(cores)
+
+
+def cpu_baseline(c2_steps=20, reps=3):
+    """FP64 oracle (OpenMP) on the host, as the CPU baseline is planned:
+    threads bound close to their cores (OMP_PROC_BIND=close,
+    OMP_PLACES=cores, set when this process starts, before any OpenMP runtime
+    reads them), one thread per core of this process's CPU share
+    (OMP_NUM_THREADS where the host sets it, else the cores this process may
+    run on); the step loop only, timed by the oracle's omp_get_wtime.
+    value = median of `reps` timings of a c2_steps-step prefix of the
+    2048x2048 precision-comparison workload; c1 = the 400x300 plain-FP64
+    workload in full (2000 steps, one timing)."""
     from oracle import binding as ob
     ob.build()
-    cores = os.cpu_count() or 1
     try:
-        cores = len(os.sched_getaffinity(0))
+        share = len(os.sched_getaffinity(0))
     except Exception:
-        pass
-    cores = min(cores, 16)
+        share = os.cpu_count() or 1
+    cores = int(os.environ.get("OMP_NUM_THREADS", share) or share)
+    cores = max(1, min(cores, share))
     ob.set_threads(cores)
-    sc = S.make_scenario(nx, ny, seed=SEED, n_slides=4, thickness=(20.0, 80.0))
-    H, _ = S.hmin_shift(sc.H)
-    W = np.ascontiguousarray(sc.state)
-    p = ob.make_params(nx, ny, sc.dx, sc.dy, angle_deg=10.0)
-    dt = ob.initial_dt(p, W, H)
-    ob.run(p, W, H, 1, dt=dt)
-    t0 = time.perf_counter()
-    ob.run(p, W, H, sample_steps, dt=dt)
-    el = time.perf_counter() - t0
-    return {"value": nx * ny * sample_steps / el / 1e6, "unit": "Mcell-updates/s", "cores": cores,
-            "kind": "port", "sample": f"FP64 oracle, {nx}x{ny} same construction, {sample_steps} steps"}
+    model, phys = host_cpu()
+
+    def rate(sc, steps, nrep):
+        H, _ = S.hmin_shift(sc.H)
+        W0 = np.ascontiguousarray(sc.state)
+        p = ob.make_params(sc.nx, sc.ny, sc.dx, sc.dy, angle_deg=10.0)
+        dt = ob.initial_dt(p, W0, H)
+        out = []
+        for _ in range(nrep):
+            W = W0.copy()
+            t0 = ob.wtime()
+            ob.run(p, W, H, steps, dt=dt, record=False)
+            out.append(sc.nx * sc.ny * steps / (ob.wtime() - t0) / 1e6)
+        return out
+
+    seed, slides = WORKLOADS[2048]
+    c2 = rate(S.make_scenario(2048, 2048, seed=seed, n_slides=slides, thickness=(20.0, 80.0)), c2_steps, reps)
+    c1 = rate(S.make_scenario(400, 300, seed=1), 2000, 1)
+    return {"value": float(np.median(c2)), "unit": "Mcell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle, 2048x2048 precision-comparison workload, {c2_steps}-step prefix, "
+                      f"median of {reps}",
+            "runs": [round(x, 3) for x in c2], "c1_400x300_2000_steps": round(c1[0], 3),
+            "cpu_model": model, "host_physical_cores": phys,
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}}
 
 
 def copy_probe(mib=256):
@@ -279,18 +323,22 @@ def main():
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=200, help="timed forward steps of the headline workload")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["both", "fp32c", "fp64"], default="both",
                     help="paths to time (profiling runs time one)")
+    ap.add_argument("--size", type=int, choices=[2048, 4096], default=HEAD_N,
+                    help="grid side of the headline forward timing (profiling runs of the 2048 workload)")
+    ap.add_argument("--c2-steps", type=int, default=200,
+                    help="timed forward steps of the 2048x2048 workload beside the headline (0 skips it)")
     ap.add_argument("--adjoint-steps", type=int, default=4000,
-                    help="steps of the adjoint workload (0 skips it)")
+                    help="steps of the 1024x1024 adjoint workload (0 skips it)")
     ap.add_argument("--adjoint-snapshots", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=200,
                     help="timed steps of the 64-scenario ensemble workload (0 skips it)")
-    ap.add_argument("--large-steps", type=int, default=100,
-                    help="timed forward steps of the 4096x4096 workload (0 skips it)")
+    ap.add_argument("--ensemble-full-steps", type=int, default=5000,
+                    help="the ensemble at the workload's full step count, no warmup (0 skips it)")
     ap.add_argument("--large-adjoint-steps", type=int, default=200,
                     help="steps of the 4096x4096 adjoint (100 HBM snapshots; the workload's full length is 20000)")
     ap.add_argument("--precision-steps", type=int, default=10000,
@@ -301,16 +349,18 @@ def main():
               "--gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
 
+    n = args.size
     fp32c = fp64 = None
     if args.precision in ("both", "fp32c"):
-        fp32c = time_gpu(L.FP32C, args.steps, args.warmup)
+        fp32c = time_gpu(L.FP32C, args.steps, args.warmup, n)
     if args.precision in ("both", "fp64"):
         fp64 = time_gpu(L.FP64, max(args.steps // 4, 10) if fp32c else args.steps,
-                        max(args.warmup // 4, 2) if fp32c else args.warmup)
+                        max(args.warmup // 4, 2) if fp32c else args.warmup, n)
     head = fp32c if fp32c else fp64
     dtype = "f32 (compensated state, f64 depth/hydrostatics/dt)" if fp32c else "f64"
-    N = NX * NY
-    achieved = BYTES_PER_CELL * N / (head["kernel_ms"] * 1e-3) / 1e9
+    seed, slides = WORKLOADS[n]
+    workload = ("large single-GPU case 4096x4096, forward step" if n == 4096
+                else "precision comparison 2048x2048, forward step")
     out = {
         "metric": "Mcell-updates/s (FP32-comp forward; plain FP64 beside it) on one MI355X",
         "value": round(head["mcells"], 2),
@@ -323,20 +373,33 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": dtype,
-        "data": "synthetic (seeded shelf + bumps + 4 slides)",
-        "config": {"workload": "precision comparison 2048x2048, forward step", "nx": NX, "ny": NY,
-                   "slides": 4, "friction_deg": 10.0, "cfl": 0.5},
+        "data": f"synthetic (seeded shelf + bumps + {slides} slides)",
+        "config": {"workload": workload, "nx": n, "ny": n, "slides": slides, "friction_deg": 10.0, "cfl": 0.5},
         "fp64_mcells": round(fp64["mcells"], 2) if fp64 else None,
         "speedup_vs_fp64": round(fp32c["mcells"] / fp64["mcells"], 3) if (fp32c and fp64) else None,
         "fp64_kernel_ms": round(fp64["kernel_ms"], 5) if fp64 else None,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": TRAFFIC_BYTES_FP32C if head is fp32c else None,
+        "fp64_achieved_gbs": round(fp64["achieved_gbs"], 1) if fp64 else None,
+        "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": TRAFFIC_BYTES_FP32C.get(n) if head is fp32c else None,
                      "kernel_ms": round(head["kernel_ms"], 5), "bytes_per_cell": BYTES_PER_CELL},
     }
     if fp32c:
         nt, ngen, split = fp32c["classes"]
         out["tile_classes"] = {"class_tiles": nt, "general_tiles": ngen, "split_launch": split}
+    if args.c2_steps > 0 and n != 2048:
+        c2 = {}
+        if fp32c:
+            r = time_gpu(L.FP32C, args.c2_steps, args.warmup, 2048)
+            c2["fp32c"] = {"mcells": round(r["mcells"], 2), "kernel_ms": round(r["kernel_ms"], 5),
+                           "achieved_gbs": round(r["achieved_gbs"], 1),
+                           "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES_FP32C[2048],
+                           "general_tiles": r["classes"][1]}
+        if fp64:
+            r = time_gpu(L.FP64, max(args.c2_steps // 4, 10), 5, 2048)
+            c2["fp64"] = {"mcells": round(r["mcells"], 2), "kernel_ms": round(r["kernel_ms"], 5),
+                          "achieved_gbs": round(r["achieved_gbs"], 1)}
+        out["c2_2048"] = c2
     if args.adjoint_steps > 0:
         adj = {}
         if fp32c:
@@ -349,10 +412,12 @@ def main():
         if fp64:
             adj["fp64"] = time_adjoint(L.FP64, args.adjoint_steps, args.adjoint_snapshots)
         out["adjoint"] = adj
+    if args.large_adjoint_steps > 0 and fp32c:
+        out["adjoint_4096"] = time_large_adjoint(args.large_adjoint_steps)
     if args.ensemble_steps > 0 and fp32c:
         out["ensemble"] = time_ensemble(args.ensemble_steps)
-    if args.large_steps > 0 and fp32c:
-        out["large_4096"] = time_large(args.large_steps, args.large_adjoint_steps)
+    if args.ensemble_full_steps > 0 and fp32c:
+        out["ensemble_full"] = time_ensemble(args.ensemble_full_steps, warmup=0)
     if args.precision_steps > 0 and fp32c and fp64:
         out["precision_2048"] = [precision_report(args.precision_steps, a) for a in (10.0, 2.0)]
     if not args.no_cpu:
