@@ -33,7 +33,8 @@ def write_case(directory: str, sc: Scenario, *, name="synthetic", prefix="out", 
                vmax=(0.0, 0.0), normalise=None, gauges_file=None) -> str:
     """Write data.dat + .bat + .ini for ``sc``; returns the data.dat path.
 
-    normalise = (L, H) selects the reference's normalised units."""
+    normalise = (L, H) selects the reference's normalised units; angle_deg is
+    one friction angle (Coulomb) or four (Pouliquen, read with --pouliquen)."""
     os.makedirs(directory, exist_ok=True)
     write_bat(os.path.join(directory, "bathy.bat"), sc)
     write_ini(os.path.join(directory, "state.ini"), sc)
@@ -44,7 +45,8 @@ def write_case(directory: str, sc: Scenario, *, name="synthetic", prefix="out", 
         lines += ["1", gauges_file]
     else:
         lines += ["0"]
-    lines += [repr(float(cfl)), repr(float(r)), repr(float(angle_deg))]
+    angles = angle_deg if isinstance(angle_deg, (tuple, list)) else (angle_deg,)
+    lines += [repr(float(cfl)), repr(float(r))] + [repr(float(a)) for a in angles]
     lines += [repr(float(x)) for x in (mfc, mf0, mfs, vmax[0], vmax[1])]
     if normalise:
         lines += ["1", repr(float(normalise[0])), repr(float(normalise[1]))]

@@ -12,6 +12,9 @@
 //   --dry-run                load, write the initial output, no GPU
 //   --dump                   print the parsed parameters as JSON and exit
 //   --quiet                  no per-step line
+//   --pouliquen              Pouliquen friction law: four friction angles in
+//                            data.dat (the reference selects the law at build
+//                            time, src/GPU/Constantes.hxx:17)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -33,7 +36,7 @@ namespace {
 void usage(const char* prog)
 {
     std::fprintf(stderr,
-                 "Uso:\n%s ficheroDatos [--precision fp64|fp32c] [--steps N] [--dry-run] [--dump] [--quiet]\n\n"
+                 "Uso:\n%s ficheroDatos [--precision fp64|fp32c] [--steps N] [--dry-run] [--dump] [--quiet] [--pouliquen]\n\n"
                  "Formato de ficheroDatos:\n"
                  "\tNombre de la batimetria\n\tLeer condiciones iniciales de fichero (0|1)\n"
                  "\tSi 0: Xmin Xmax Ymin Ymax Volumenes_x Volumenes_y\n"
@@ -41,7 +44,7 @@ void usage(const char* prog)
                  "\tBorde superior, inferior, izquierdo, derecho (1: abierto, 0 o -1: pared)\n"
                  "\tTiempo de simulacion\n\tTiempo de guardado (-1: no guardar, 0: todo)\n"
                  "\tLeer fichero de puntos (0|1) [fichero]\n\tCFL\n\tRatio de densidades\n"
-                 "\tAngulo de reposo\n\tFriccion entre capas, agua-fondo, sedimentos-fondo\n"
+                 "\tAngulo de reposo (--pouliquen: cuatro angulos)\n\tFriccion entre capas, agua-fondo, sedimentos-fondo\n"
                  "\tVelocidad maxima agua, sedimentos\n\tNormalizar (0|1) [L H]\n"
                  "\tPrefijo de los ficheros de guardado\n",
                  prog);
@@ -128,9 +131,17 @@ struct Output {
             w.put_att_float(-1, "simulation_time", (float)(pb.tiempo_tot * pb.T));
             w.put_att_float(-1, "CFL", (float)p.cfl);
             w.put_att_float(-1, "ratio_of_densities", (float)p.r);
-            w.put_att_text(-1, "friction_law", "Coulomb");
-            char a[64];
-            std::snprintf(a, sizeof a, "%.4f", p.angle1 * 180.0 / M_PI);
+            // friction law and angles in degrees as src/GPU/netcdf.cu:236-244 writes them
+            char a[128];
+            const double d = 180.0 / M_PI;
+            if (p.friction_law == 1) {
+                w.put_att_text(-1, "friction_law", "Pouliquen");
+                std::snprintf(a, sizeof a, "%.4f, %.4f, %.4f, %.4f", (float)(p.angle1 * d), (float)(p.angle2 * d),
+                              (float)(p.angle3 * d), (float)(p.angle4 * d));
+            } else {
+                w.put_att_text(-1, "friction_law", "Coulomb");
+                std::snprintf(a, sizeof a, "%.4f", (float)(p.angle1 * d));
+            }
             w.put_att_text(-1, "angle_of_repose", a);
             w.put_att_float(-1, "friction_between_layers", (float)(p.mfc / pb.L));
             w.put_att_float(-1, "friction_water_bottom", (float)(p.mf0 / fac));
@@ -215,6 +226,7 @@ int main(int argc, char** argv)
     int precision = LHYSEA_FP64;
     long fixed_steps = -1;
     bool dry = false, dump = false, quiet = false;
+    int friction_law = 0;
     for (int i = 2; i < argc; i++) {
         std::string a = argv[i];
         if (a == "--precision" && i + 1 < argc) {
@@ -230,6 +242,8 @@ int main(int argc, char** argv)
             dump = true;
         } else if (a == "--quiet") {
             quiet = true;
+        } else if (a == "--pouliquen") {
+            friction_law = 1;
         } else {
             usage(argv[0]);
             return 1;
@@ -237,7 +251,7 @@ int main(int argc, char** argv)
     }
     Problem pb;
     std::string msg;
-    if (load_problem(data, pb, msg) != 0) {
+    if (load_problem(data, pb, msg, friction_law) != 0) {
         std::fprintf(stderr, "%s\n", msg.c_str());
         return 1;
     }
@@ -248,12 +262,15 @@ int main(int argc, char** argv)
                     "\"xmin\": %.17g, \"xmax\": %.17g, \"ymin\": %.17g, \"ymax\": %.17g, \"dx\": %.17g, \"dy\": %.17g, "
                     "\"borde\": [%.17g, %.17g, %.17g, %.17g], \"tiempo_tot\": %.17g, \"tiempo_guardar\": %.17g, "
                     "\"leer_fichero_puntos\": %d, \"cfl\": %.17g, \"r\": %.17g, \"angle1_deg\": %.17g, "
+                    "\"friction_law\": %d, \"angles_deg\": [%.17g, %.17g, %.17g, %.17g], "
                     "\"mfc\": %.17g, \"mf0\": %.17g, \"mfs\": %.17g, \"vmax1\": %.17g, \"vmax2\": %.17g, "
                     "\"normalizar\": %d, \"L\": %.17g, \"H\": %.17g, \"Q\": %.17g, \"T\": %.17g, \"g\": %.17g, "
                     "\"eps_h\": %.17g, \"Hmin\": %.17g, \"gauges\": %zu}\n",
                     pb.nombre_bati.c_str(), pb.prefijo.c_str(), p.nx, p.ny, pb.xmin, pb.xmax, pb.ymin, pb.ymax,
                     p.dx, p.dy, p.borde_sup, p.borde_inf, p.borde_izq, p.borde_der, pb.tiempo_tot,
-                    pb.tiempo_guardar, pb.leer_fichero_puntos, p.cfl, p.r, p.angle1 * 180.0 / M_PI, p.mfc, p.mf0,
+                    pb.tiempo_guardar, pb.leer_fichero_puntos, p.cfl, p.r, p.angle1 * 180.0 / M_PI, p.friction_law,
+                    p.angle1 * 180.0 / M_PI, p.angle2 * 180.0 / M_PI, p.angle3 * 180.0 / M_PI,
+                    p.angle4 * 180.0 / M_PI, p.mfc, p.mf0,
                     p.mfs, p.vmax1, p.vmax2, pb.normalizar, pb.L, pb.H, pb.Q, pb.T, p.g, p.eps_h, pb.Hmin,
                     pb.gauge_index.size());
         return 0;

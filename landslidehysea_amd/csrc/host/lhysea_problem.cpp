@@ -50,7 +50,7 @@ int point_index(const std::vector<float>& X, const std::vector<float>& Y, float 
 
 }  // namespace
 
-int load_problem(const std::string& fich_ent, Problem& pb, std::string& msg)
+int load_problem(const std::string& fich_ent, Problem& pb, std::string& msg, int friction_law)
 {
     // directory of data.dat: relative file names are resolved against it
     size_t slash = fich_ent.find_last_of("/\\");
@@ -97,10 +97,17 @@ int load_problem(const std::string& fich_ent, Problem& pb, std::string& msg)
         }
     }
     f >> p.cfl >> p.r;
-    double a1 = 0;
-    f >> a1;
-    p.angle1 = p.angle2 = p.angle3 = p.angle4 = a1 * M_PI / 180.0;
-    p.friction_law = 0;
+    // Coulomb: one angle; Pouliquen: four (src/GPU/Problema.cxx:240-256, chosen
+    // at build time there by COULOMB in src/GPU/Constantes.hxx:17)
+    double a[4] = {0, 0, 0, 0};
+    f >> a[0];
+    if (friction_law == 1) f >> a[1] >> a[2] >> a[3];
+    else a[1] = a[2] = a[3] = a[0];
+    p.angle1 = a[0] * M_PI / 180.0;
+    p.angle2 = a[1] * M_PI / 180.0;
+    p.angle3 = a[2] * M_PI / 180.0;
+    p.angle4 = a[3] * M_PI / 180.0;
+    p.friction_law = friction_law;
     f >> p.mfc >> p.mf0 >> p.mfs >> p.vmax1 >> p.vmax2;
     f >> pb.normalizar;
     if (pb.normalizar == 0) {
@@ -220,7 +227,11 @@ std::string describe_problem(const Problem& pb)
     o << "Y: [" << pb.ymin * pb.L << ", " << pb.ymax * pb.L << "]\n";
     o << "CFL: " << p.cfl << "\n";
     o << "r: " << p.r << "\n";
-    o << "Angulo de reposo: " << p.angle1 * 180.0 / M_PI << "\n";
+    if (p.friction_law == 1)
+        o << "Angulos de reposo: " << p.angle1 * 180.0 / M_PI << ", " << p.angle2 * 180.0 / M_PI << ", "
+          << p.angle3 * 180.0 / M_PI << ", " << p.angle4 * 180.0 / M_PI << "\n";
+    else
+        o << "Angulo de reposo: " << p.angle1 * 180.0 / M_PI << "\n";
     o << "Friccion entre capas: " << p.mfc / pb.L << "\n";
     o << "Friccion agua-fondo: " << p.mf0 / fac << "\n";
     o << "Friccion sedimento-fondo: " << p.mfs / fac << "\n";

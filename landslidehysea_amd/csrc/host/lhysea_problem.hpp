@@ -24,8 +24,9 @@ struct Problem {
 };
 
 // Returns 0 on success, 1 if a file is missing or unreadable (the reference's
-// error code); msg receives a description.
-int load_problem(const std::string& data_dat, Problem& pb, std::string& msg);
+// error code); msg receives a description.  friction_law 0 (Coulomb) reads one
+// friction angle, 1 (Pouliquen) four.
+int load_problem(const std::string& data_dat, Problem& pb, std::string& msg, int friction_law = 0);
 
 // Parameter echo in the reference's format (src/GPU/Problema.cxx:556-580).
 std::string describe_problem(const Problem& pb);

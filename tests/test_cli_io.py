@@ -135,3 +135,29 @@ def test_gauge_text_output(tmp_path):
     for k, (i, j) in enumerate(((3, 4), (10, 12))):
         # reference gauge value: (h1 - H - Hmin) * H_scale in the shifted frame = h1 - H
         assert float(line[1 + k]) == pytest.approx(sc.h1[j, i] - H[j, i], abs=1e-6)
+
+
+def test_pouliquen_grammar_and_netcdf_attributes(tmp_path):
+    """--pouliquen reads four friction angles (src/GPU/Problema.cxx:247-255,
+    the reference's build without COULOMB) and the fields after them stay in
+    place; the NetCDF files name the law and its angles (src/GPU/netcdf.cu:236-244)."""
+    from scipy.io import netcdf_file
+    sc = S.make_scenario(30, 20, seed=9, semi_axes=(6.0, 4.0))
+    path = write_case(str(tmp_path), sc, prefix="pq", t_save=0.0, angle_deg=(12.0, 25.0, 13.5, 30.0),
+                      mfc=0.07, vmax=(3.0, 4.0))
+    r = run([os.path.basename(path), "--dump", "--pouliquen"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    assert d["friction_law"] == 1
+    assert d["angles_deg"] == pytest.approx([12.0, 25.0, 13.5, 30.0], rel=1e-14)
+    assert d["mfc"] == 0.07 and d["vmax1"] == 3.0 and d["vmax2"] == 4.0 and d["prefijo"] == "pq"
+    r = run([os.path.basename(path), "--dry-run", "--pouliquen"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "Angulos de reposo: 12, 25, 13.5, 30" in r.stdout
+    with netcdf_file(str(tmp_path / "pq_eta1.nc"), "r", mmap=False) as f:
+        assert f.friction_law == b"Pouliquen"
+        assert f.angle_of_repose == b"12.0000, 25.0000, 13.5000, 30.0000"
+    # the same file read as Coulomb takes the second angle for mfc: the grammar is positional
+    r = run([os.path.basename(path), "--dump"], tmp_path)
+    d = json.loads(r.stdout)
+    assert d["friction_law"] == 0 and d["mfc"] == 25.0

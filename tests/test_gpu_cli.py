@@ -35,10 +35,12 @@ def cli(args, cwd):
 
 
 def params_of(dump, precision=L.FP64):
+    law = dump.get("friction_law", 0)
     return L.Params(nx=dump["nx"], ny=dump["ny"], dx=dump["dx"], dy=dump["dy"], borde=tuple(dump["borde"]),
-                    cfl=dump["cfl"], r=dump["r"], angle_deg=(dump["angle1_deg"],), mfc=dump["mfc"],
+                    cfl=dump["cfl"], r=dump["r"],
+                    angle_deg=tuple(dump["angles_deg"]) if law == 1 else (dump["angle1_deg"],), mfc=dump["mfc"],
                     mf0=dump["mf0"], mfs=dump["mfs"], vmax1=dump["vmax1"], vmax2=dump["vmax2"], g=dump["g"],
-                    eps_h=dump["eps_h"], L=dump["L"], H=dump["H"], precision=precision)
+                    eps_h=dump["eps_h"], L=dump["L"], H=dump["H"], friction_law=law, precision=precision)
 
 
 def library_frames(prm, W, H, t_end, t_save):
@@ -63,14 +65,20 @@ def library_frames(prm, W, H, t_end, t_save):
 # --quiet steps to each save time in one call (run_until); without it the
 # executable steps one at a time and prints a line per step; a save interval
 # shorter than dt saves after every step
-@pytest.mark.parametrize("precision,t_end,t_save,quiet", [("fp64", 12.0, 3.0, True), ("fp32c", 12.0, 3.0, True),
-                                                          ("fp64", 2.0, 0.05, True), ("fp32c", 12.0, 3.0, False)])
-def test_cli_netcdf_run_matches_library(tmp_path, precision, t_end, t_save, quiet):
+# --pouliquen: the Pouliquen law with four angles read from data.dat
+@pytest.mark.parametrize("precision,t_end,t_save,quiet,pouliquen",
+                         [("fp64", 12.0, 3.0, True, False), ("fp32c", 12.0, 3.0, True, False),
+                          ("fp64", 2.0, 0.05, True, False), ("fp32c", 12.0, 3.0, False, False),
+                          ("fp64", 12.0, 3.0, True, True), ("fp32c", 12.0, 3.0, True, True)])
+def test_cli_netcdf_run_matches_library(tmp_path, precision, t_end, t_save, quiet, pouliquen):
     from scipy.io import netcdf_file
     sc = S.make_scenario(64, 48, seed=1, semi_axes=(12.0, 9.0))
-    write_case(str(tmp_path), sc, prefix="res", t_end=t_end, t_save=t_save, angle_deg=2.0)
-    dump = json.loads(cli(["data.dat", "--dump"], tmp_path).stdout)
-    r = cli(["data.dat", "--precision", precision] + (["--quiet"] if quiet else []), tmp_path)
+    law = ["--pouliquen"] if pouliquen else []
+    write_case(str(tmp_path), sc, prefix="res", t_end=t_end, t_save=t_save,
+               angle_deg=(2.0, 6.0, 3.0, 8.0) if pouliquen else 2.0)
+    dump = json.loads(cli(["data.dat", "--dump"] + law, tmp_path).stdout)
+    assert dump["friction_law"] == (1 if pouliquen else 0)
+    r = cli(["data.dat", "--precision", precision] + (["--quiet"] if quiet else []) + law, tmp_path)
     say(f"L-HySEA.exe ({precision}) rc={r.returncode}: {r.stdout.strip().splitlines()[-1] if r.stdout else ''}")
     assert r.returncode == 0, r.stderr
     H, _ = S.hmin_shift(sc.H)
@@ -81,6 +89,7 @@ def test_cli_netcdf_run_matches_library(tmp_path, precision, t_end, t_save, quie
         eta = f.variables["eta1"][:].copy()
         t_nc = f.variables["time"][:].copy()
         mh = f.variables["max_height"][:].copy()
+        assert f.friction_law == (b"Pouliquen" if pouliquen else b"Coulomb")
     with netcdf_file(str(tmp_path / "res_q2x.nc"), "r", mmap=False) as f:
         q2x = f.variables["q2x"][:].copy()
     say(f"  frames {len(t_nc)} at {t_nc.tolist()}; library {len(frames)} at {[round(t, 4) for t in times]}")
