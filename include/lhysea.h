@@ -42,6 +42,12 @@ extern "C" {
 #define LHYSEA_ERR_HIP 4
 #define LHYSEA_ERR_NO_DEVICE 5
 #define LHYSEA_ERR_IO 6
+#define LHYSEA_ERR_NUMERIC 7   /* the state went bad: see lhysea_health */
+
+/* health bits of a state (lhysea_health, lhysea_batch_health) */
+#define LHYSEA_HEALTH_NONFINITE 1       /* a cell update produced a NaN or infinity */
+#define LHYSEA_HEALTH_NEGATIVE_DEPTH 2  /* a layer depth below -eps_h after the flux
+                                           update, before the clip to zero */
 
 /* arithmetic of the step */
 #define LHYSEA_FP64 0   /* plain FP64 */
@@ -89,13 +95,25 @@ int lhysea_download(lhysea_sim *s, double *W);
 int lhysea_initial_dt(lhysea_sim *s, double *dt);
 /* Force the dt of the next step. */
 int lhysea_set_dt(lhysea_sim *s, double dt);
-/* Run nsteps steps asynchronously on the handle's stream (no host sync). */
+/* Run nsteps steps asynchronously on the handle's stream (the host reads the
+ * control block before each piece of at most 1024 steps, not after the last).
+ * Returns LHYSEA_ERR_NUMERIC, launching nothing more, when the health word
+ * (lhysea_health) shows a bad state at one of those reads. */
 int lhysea_step(lhysea_sim *s, int nsteps);
 /* Step while t < t_end (the reference's loop condition, src/GPU/ShallowWater.cu:269)
  * and at most max_steps steps; *steps_done (may be NULL) gets the steps run.
  * The fused kernels stop on the device, so the host waits once per chunk of
- * steps, not once per step. */
+ * at most 512 steps, not once per step; a chunk that leaves the health word
+ * set ends the run with LHYSEA_ERR_NUMERIC (*steps_done is still set). */
 int lhysea_run_until(lhysea_sim *s, double t_end, int32_t max_steps, int32_t *steps_done);
+/* Failure detection (no counterpart in the reference, which has no NaN
+ * checks): every cell update of the step kernels checks its new state, and a
+ * workgroup that finds a non-finite value or a depth below -eps_h raises the
+ * corresponding LHYSEA_HEALTH_* bit in the handle's control block, with the
+ * first step that raised one.  *flags gets the bits raised since the last
+ * upload (0: healthy), *first_step (may be NULL) that step (-1 if none).
+ * Waits for the handle's stream. */
+int lhysea_health(lhysea_sim *s, uint32_t *flags, int32_t *first_step);
 int lhysea_sync(lhysea_sim *s);
 /* time at the current state, dt of the next step, completed steps */
 int lhysea_get_time(lhysea_sim *s, double *t, double *dt_next, int32_t *steps);
@@ -194,6 +212,8 @@ int lhysea_batch_step(lhysea_batch *b, int32_t nsteps);
 int lhysea_batch_step_eager(lhysea_batch *b, int32_t nsteps);
 int lhysea_batch_sync(lhysea_batch *b);
 int lhysea_batch_get_time(lhysea_batch *b, int32_t sid, double *t, double *dt_next, int32_t *steps);
+/* lhysea_health of scenario sid (raised by the batch's step kernels). */
+int lhysea_batch_health(lhysea_batch *b, int32_t sid, uint32_t *flags, int32_t *first_step);
 int lhysea_batch_track_eta_max(lhysea_batch *b, int32_t sid, int enable);
 int lhysea_batch_eta_max(lhysea_batch *b, int32_t sid, double *val, double *time);
 int lhysea_batch_set_gauges(lhysea_batch *b, int32_t sid, const int32_t *cells, int32_t ng, int32_t max_steps);
@@ -229,7 +249,9 @@ int lhysea_batch_autotune(lhysea_batch *b, int32_t nsteps, double *ms_per_tile, 
  * fits), larger ones are refused with LHYSEA_ERR_ARGUMENT.  Calls that
  * overlap on one GPU share that array and race; run them one after another.
  * The call waits for its stream once after the tile classification and once
- * at the end (it reads the control block back). */
+ * at the end (it reads the control block back); it returns
+ * LHYSEA_ERR_NUMERIC (outputs set) when a step raised a health bit
+ * (lhysea_health). */
 int lhysea_step_device(const lhysea_params *p, const double *H, void *W, void *W_work, int32_t nsteps,
                        double *t, double *dt, int32_t *result_in_work, void *stream);
 

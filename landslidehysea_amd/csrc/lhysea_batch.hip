@@ -539,6 +539,17 @@ int lhysea_batch_get_gauges(lhysea_batch* b, int32_t sid, double* out, int32_t* 
     return LHYSEA_OK;
 }
 
+int lhysea_batch_health(lhysea_batch* b, int32_t sid, uint32_t* flags, int32_t* first_step)
+{
+    if (bad_sid(b, sid) || !flags) return bfail(LHYSEA_ERR_ARGUMENT, "bad arguments");
+    Ctl c;
+    BHIPCHK(hipStreamSynchronize(b->stream));
+    BHIPCHK(hipMemcpy(&c, b->ctl + sid, sizeof(Ctl), hipMemcpyDeviceToHost));
+    *flags = c.health;
+    if (first_step) *first_step = c.health ? c.health_step : -1;
+    return LHYSEA_OK;
+}
+
 int lhysea_batch_dt_history(lhysea_batch* b, int32_t sid, double* out, int32_t max_steps, int32_t* nrec)
 {
     if (bad_sid(b, sid) || max_steps < 0) return bfail(LHYSEA_ERR_ARGUMENT, "bad arguments");

@@ -16,15 +16,20 @@ namespace {
 
 thread_local std::string g_err;
 
+// Every failure return records its message and clears HIP's per-thread last
+// error: a failed runtime call (an allocation beyond the device's memory, say)
+// leaves that error set, and the next kernel-launch check would report it.
 int fail(int code, const std::string& msg)
 {
     g_err = msg;
+    (void)hipGetLastError();
     return code;
 }
 
 int hip_fail(hipError_t e, const char* where)
 {
     g_err = std::string(where) + ": " + hipGetErrorString(e);
+    (void)hipGetLastError();
     return (e == hipErrorOutOfMemory) ? LHYSEA_ERR_GPU_MEMORY : LHYSEA_ERR_HIP;
 }
 
@@ -36,7 +41,20 @@ int hip_fail(hipError_t e, const char* where)
 
 }  // namespace
 
-void lhysea::set_last_error(const std::string& msg) { g_err = msg; }
+void lhysea::set_last_error(const std::string& msg)
+{
+    g_err = msg;
+    (void)hipGetLastError();
+}
+
+int lhysea::numeric_fail(const Ctl& c)
+{
+    std::string m = "bad state from step " + std::to_string(c.health_step) + ":";
+    if (c.health & HEALTH_NONFINITE) m += " non-finite value";
+    if (c.health & HEALTH_NEGATIVE_DEPTH) m += " depth below -eps_h";
+    g_err = m;
+    return LHYSEA_ERR_NUMERIC;
+}
 
 int lhysea::check_params(const lhysea_params& p)
 {
@@ -404,6 +422,7 @@ int lhysea_step(lhysea_sim* s, int nsteps)
     Ctl c;
     HIPCHK(hipMemcpyAsync(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    if (c.health) return numeric_fail(c);
     if (!(c.dt_cur > 0) && !s->dt_seq) {
         int rc = lhysea_initial_dt(s, nullptr);
         if (rc) return rc;
@@ -413,6 +432,7 @@ int lhysea_step(lhysea_sim* s, int nsteps)
         if (done > 0) {
             HIPCHK(hipMemcpyAsync(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
             HIPCHK(hipStreamSynchronize(s->stream));
+            if (c.health) return numeric_fail(c);
         }
         choose_split(s, c);
         const int n = (nsteps - done < 1024) ? nsteps - done : 1024;
@@ -436,6 +456,8 @@ int lhysea_run_until(lhysea_sim* s, double t_end, int32_t max_steps, int32_t* st
         HIPCHK(hipMemcpy(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
     }
     const int step0 = c.step;
+    if (steps_done) *steps_done = 0;
+    if (c.health) return numeric_fail(c);
     const bool device_stop = (s->mode == MODE_FUSED);
     while (c.t < t_end && c.step - step0 < max_steps) {
         // chunks sized from the current dt; steps launched after t reaches
@@ -443,7 +465,7 @@ int lhysea_run_until(lhysea_sim* s, double t_end, int32_t max_steps, int32_t* st
         int chunk = 1;
         if (device_stop) {
             const double est = (c.dt_cur > 0) ? (t_end - c.t) / c.dt_cur : 1.0;
-            chunk = (est > 511.0) ? 512 : (int)est + 1;
+            chunk = (est > 511.0) ? 512 : (est >= 0.0) ? (int)est + 1 : 1;
         }
         if (chunk > max_steps - (c.step - step0)) chunk = max_steps - (c.step - step0);
         choose_split(s, c);
@@ -454,8 +476,23 @@ int lhysea_run_until(lhysea_sim* s, double t_end, int32_t max_steps, int32_t* st
         HIPCHK(hipMemcpy(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
         // the state is in the buffer the last executed step wrote
         s->cur = cur0 ^ ((c.step - stepc) & 1);
+        if (c.health) {
+            if (steps_done) *steps_done = c.step - step0;
+            return numeric_fail(c);
+        }
     }
     if (steps_done) *steps_done = c.step - step0;
+    return LHYSEA_OK;
+}
+
+int lhysea_health(lhysea_sim* s, uint32_t* flags, int32_t* first_step)
+{
+    if (!s || !flags) return fail(LHYSEA_ERR_ARGUMENT, "null argument");
+    Ctl c;
+    HIPCHK(hipMemcpyAsync(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    *flags = c.health;
+    if (first_step) *first_step = c.health ? c.health_step : -1;
     return LHYSEA_OK;
 }
 

@@ -41,7 +41,26 @@ struct Ctl {
     // step at which the class tables were last rebuilt from a state (upload,
     // restore); the tables describe every step from it on
     int cls_from = 0x7fffffff;
+    // health of the state: HEALTH_* bits raised by the cell updates since the
+    // state was set, and the first step that raised one
+    unsigned int health = 0u;
+    int health_step = 0x7fffffff;
 };
+
+// Health bits (the C-ABI's LHYSEA_HEALTH_*): a non-finite value in a new
+// state, a layer depth below -eps_h after the flux update (before the clip to
+// zero; the positivity limiter keeps depths at or above zero).
+constexpr unsigned HEALTH_NONFINITE = 1u, HEALTH_NEGATIVE_DEPTH = 2u;
+
+// Records this thread's health bits for step s (nothing when none is set, the
+// common case: the branch is skipped).
+__device__ __forceinline__ void raise_health(Ctl* ctl, unsigned bits, int s)
+{
+    if (bits && ctl) {
+        atomicOr(&ctl->health, bits);
+        atomicMin(&ctl->health_step, s);
+    }
+}
 
 // dt from the wave-speed max: min_i dt_i with dt_i = 2 CFL area / A_i, or 1e30
 // when A_i < EPSILON (src/GPU/Volumen_kernel.cu:215-229, 255-257); the

@@ -482,8 +482,10 @@ struct KCell {
 };
 
 // Cell update (src/GPU/Volumen_kernel.cu:7-180, 231-284) on compensated
-// pairs: o = old state (h1,q1x,q1y,h2,q2x,q2y), acc = flux sums.
-__device__ __forceinline__ void cell_update(const KCell& k, Pair* o, const float* acc)
+// pairs: o = old state (h1,q1x,q1y,h2,q2x,q2y), acc = flux sums.  hb (if
+// given) collects the health bits: a depth below -eps_h before the clip, a
+// non-finite new state.
+__device__ __forceinline__ void cell_update(const KCell& k, Pair* o, const float* acc, unsigned* hb = nullptr)
 {
     const float eps = k.eps, eps4 = k.eps4, dt = k.dtf;
     // old-state velocities for the implicit friction (speeds taken where needed)
@@ -494,6 +496,7 @@ __device__ __forceinline__ void cell_update(const KCell& k, Pair* o, const float
 #pragma unroll
     for (int f = 0; f < 6; f++) n[f] = add(o[f], k.valf * acc[f]);
 
+    if (hb && (n[0].hi < -eps || n[3].hi < -eps)) *hb |= HEALTH_NEGATIVE_DEPTH;
     // filtroEstado: clip, damp thin layers, speed caps
     if (negative(n[0])) n[0] = Pair{0.0f, 0.0f};
     if (negative(n[3])) n[3] = Pair{0.0f, 0.0f};
@@ -590,6 +593,8 @@ __device__ __forceinline__ void cell_update(const KCell& k, Pair* o, const float
         float f = ((nq + sc >= LH_EPSF) && (n[3].hi > 0.0f) && (nq >= sc)) ? nq * rcp(nq + sc) : 0.0f;
         if (f != 1.0f) { n[4] = scale(n[4], f); n[5] = scale(n[5], f); }
     }
+    if (hb && !(__builtin_fabsf(((n[0].hi + n[1].hi) + (n[2].hi + n[3].hi)) + (n[4].hi + n[5].hi)) <= 3.40282347e38f))
+        *hb |= HEALTH_NONFINITE;
 #pragma unroll
     for (int f = 0; f < 6; f++) o[f] = n[f];
 }
@@ -599,7 +604,7 @@ __device__ __forceinline__ void cell_update(const KCell& k, Pair* o, const float
 // and the Coulomb scaling multiply zeros, and disImplicita can only take its
 // water-bottom case, so those steps are dropped; the layer-1 steps are
 // cell_update()'s, in its order.  o[3..5] are left as they are (zero).
-__device__ __forceinline__ void cell_update1(const KCell& k, Pair* o, const float* acc)
+__device__ __forceinline__ void cell_update1(const KCell& k, Pair* o, const float* acc, unsigned* hb = nullptr)
 {
     const float eps = k.eps, eps4 = k.eps4, dt = k.dtf;
     const float so1 = sdes(o[0].hi, eps, eps4);
@@ -609,6 +614,7 @@ __device__ __forceinline__ void cell_update1(const KCell& k, Pair* o, const floa
 #pragma unroll
     for (int f = 0; f < 3; f++) n[f] = add(o[f], k.valf * acc[f]);
 
+    if (hb && n[0].hi < -eps) *hb |= HEALTH_NEGATIVE_DEPTH;
     if (negative(n[0])) n[0] = Pair{0.0f, 0.0f};
     {
         const float lim = dt * 6.7108864e7f;
@@ -639,6 +645,7 @@ __device__ __forceinline__ void cell_update1(const KCell& k, Pair* o, const floa
             n[2] = scale(n[2], f);
         }
     }
+    if (hb && !(__builtin_fabsf((n[0].hi + n[1].hi) + n[2].hi) <= 3.40282347e38f)) *hb |= HEALTH_NONFINITE;
 #pragma unroll
     for (int f = 0; f < 3; f++) o[f] = n[f];
 }

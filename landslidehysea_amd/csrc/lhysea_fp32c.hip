@@ -188,6 +188,8 @@ __device__ __forceinline__ int halo_mask(int lx, int ly) { return halo_mask_of<F
 // smask (optional): OR of halo_mask() over this thread's cells that keep
 // sediment (h2 or q2 nonzero) in the new state, with the cell's position in
 // the FTX x FTY class tile at (cx0, cy0).
+// hc (optional): control block whose health word records the cell updates'
+// health bits (HEALTH_*) as those of step hs.
 // RECOMP: instead of updating the state, write the partial h1/h2 sums that
 // classes 2..4 read (P, [3][2][N]) and the final flux sums (ACC, [6][N]) of
 // the tile's cells: the forward recomputation of the discrete adjoint.
@@ -197,7 +199,8 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
                                             double* __restrict__ emax, int i0, int j0,
                                             float* __restrict__ P = nullptr, float* __restrict__ ACC = nullptr,
                                             int xend = 0x7fffffff, GaugeOut go = GaugeOut{nullptr, 0, nullptr},
-                                            int* smask = nullptr, int cx0 = 0, int cy0 = 0)
+                                            int* smask = nullptr, int cx0 = 0, int cy0 = 0, Ctl* hc = nullptr,
+                                            int hs = 0)
 {
     static_assert(TX % 2 == 0 && TY % 2 == 0, "tile origins must stay even");
     // LDS rows are padded to an odd length (CW + 1 words) so that the
@@ -443,6 +446,7 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
         const int r = e / TX, c = e % TX;
         const int gj = j0 + r, gi = i0 + c;
         if (gj >= ny || gi >= nx || gi >= xend) continue;
+        unsigned hb = 0u;   // raised per cell: nothing stays live across the loop
         const size_t g = (size_t)gj * nx + gi;
         Pair o[6];
         const double h1d = sh1[r + 1][c + 1], h2d = sh2[r + 1][c + 1];
@@ -465,7 +469,7 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
         const bool bare = (h2d == 0.0) && (sq[2][r + 1][c + 1] == 0.0f) && (sq[3][r + 1][c + 1] == 0.0f) &&
                           (ac[3] == 0.0f) && (ac[4] == 0.0f) && (ac[5] == 0.0f);
         if (__all(bare)) {
-            cell_update1(kk, o, ac);
+            cell_update1(kk, o, ac, &hb);
 #pragma unroll
             for (int f = 0; f < 3; f++) {
                 Wn[f * N + g] = o[f].hi;
@@ -480,7 +484,7 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
             o[3].hi = (float)h2d; o[3].lo = (float)(h2d - (double)o[3].hi);
             o[4] = Pair{sq[2][r + 1][c + 1], W[10 * N + g]};
             o[5] = Pair{sq[3][r + 1][c + 1], W[11 * N + g]};
-            cell_update(kk, o, ac);
+            cell_update(kk, o, ac, &hb);
 #pragma unroll
             for (int f = 0; f < 6; f++) {
                 Wn[f * N + g] = o[f].hi;
@@ -493,6 +497,7 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
             if (smask && (o[3].hi != 0.0f || o[4].hi != 0.0f || o[5].hi != 0.0f))
                 *smask |= halo_mask(gi - cx0, gj - cy0);
         }
+        raise_health(hc, hb, hs);
     }
     if (go.out) gauges_of_tile<TX, TY, NT>(go, &sh1[0][0], &sh2[0][0], &sH[0][0], CWP, nx, i0, j0, xend);
     return Amax;
@@ -510,12 +515,14 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
 // From step skip_from + 2 on, Wn holds the state that step s - 2 wrote; a
 // tile that step took as sediment-free (its entry, stamped for s - 2 and not
 // raised to s, differs from s - 2) has zero layer-2 planes there already, so
-// only the three layer-1 planes and their compensations are written.
+// only the three layer-1 planes and their compensations are written.  hc
+// (optional): control block whose health word records the cell updates'
+// health bits as those of step s.
 template <int TX, int TY, int NT>
 __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __restrict__ W, float* __restrict__ Wn,
                                                const double* __restrict__ H, double dt, double tnow,
                                                double* __restrict__ emax, int i0, int j0, const int* cls, int s,
-                                               int skip_from, GaugeOut go, double& Amax)
+                                               int skip_from, GaugeOut go, double& Amax, Ctl* hc = nullptr)
 {
     static_assert(TX % 2 == 0 && TY % 2 == 0, "tile origins must stay even");
     constexpr int CW = TX + 2, RW = TY + 2, CWP = CW + 1;
@@ -689,6 +696,7 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
     // cell update of the tile's own cells (layer 2 stays zero)
     const KCell kk = make_kcell(k, kc, dt);
     Amax = 0.0;
+    unsigned hb = 0u;
 #pragma unroll
     for (int it = 0; it < CU_IT; it++) {
         const int e = tid + it * NT;
@@ -711,7 +719,7 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
                 emax[2 * g + 1] = tnow;
             }
         }
-        cell_update1(kk, o, ac);
+        cell_update1(kk, o, ac, &hb);
 #pragma unroll
         for (int f = 0; f < 3; f++) {
             Wn[f * N + g] = o[f].hi;
@@ -727,6 +735,7 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
         // the new h1 replaces this cell's staged one (read by no other thread now)
         if (go.out) sh1[r + 1][c + 1] = (double)o[0].hi + (double)o[0].lo;
     }
+    raise_health(hc, hb, s);
     if (go.out) gauges_of_tile<TX, TY, NT>(go, &sh1[0][0], nullptr, &sH[0][0], CWP, nx, i0, j0, 0x7fffffff);
     return true;
 }
@@ -757,7 +766,7 @@ __global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const float* __rest
     const GaugeOut go{a.gauges, a.ng, (a.gseries && s < a.gcap) ? a.gseries + (size_t)s * a.ng : nullptr};
     double Amax;
     if (!tile_step_bare<TX, TY, NT>(k, W, Wn, H, dt, a.ctl->t, a.emax, tx * TX, ty * TY, tc.cls + (s & 1) * nb + b,
-                                    s, a.wn_prev ? a.ctl->cls_from : 0x7fffffff, go, Amax))
+                                    s, a.wn_prev ? a.ctl->cls_from : 0x7fffffff, go, Amax, a.ctl))
         return;
     block_max_atomic<NT>(Amax, &a.ctl->maxA_next);
 }
@@ -775,7 +784,7 @@ __global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const float* __rest
 // closing workgroup reads the max with an atomic.  The sediment-free launch
 // ran before this one, so its maxima are in place.
 template <int STX, int TY, int NT>
-__global__ void __launch_bounds__(NT) k_fused_gen(Consts k, const float* __restrict__ W, float* __restrict__ Wn,
+__global__ void __launch_bounds__(NT, 3) k_fused_gen(Consts k, const float* __restrict__ W, float* __restrict__ Wn,
                                                   const double* __restrict__ H, StepArgs a, TileCls tc)
 {
     constexpr int NSUB = (FTX + STX - 1) / STX;
@@ -800,7 +809,7 @@ __global__ void __launch_bounds__(NT) k_fused_gen(Consts k, const float* __restr
         const int cx0 = tx * FTX, cy0 = ty * FTY;
         int m = 0;
         Amax = fmax(Amax, tile_step<STX, TY, NT>(k, W, Wn, H, dt, tnow, a.emax, cx0 + sub * STX, cy0, nullptr,
-                                                 nullptr, cx0 + FTX, go, &m, cx0, cy0));
+                                                 nullptr, cx0 + FTX, go, &m, cx0, cy0, ctl, s));
         // also orders this sub-tile's LDS reads before the next one's staging
         m = block_or(m);
         if (m && threadIdx.x < 64) mark_general(tc, ctl, tx, ty, s + 1, m);
@@ -813,7 +822,7 @@ __global__ void __launch_bounds__(NT) k_fused_gen(Consts k, const float* __restr
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < NT / 64; w++) v = fmax(v, red[w]);
-        v = fmax(v, red[0]);
+        v = fabs(fmax(v, red[0]));   // sign bit clear also for a NaN from a corrupt state
         const unsigned long long old = atomicMax(&ctl->maxA_next, (unsigned long long)__double_as_longlong(v));
         // bit 63 of a non-negative double is clear: the increment is 1
         const unsigned done = atomicAdd(&ctl->blocks_done, 1u + (unsigned)(old >> 63));
@@ -853,7 +862,7 @@ __global__ void __launch_bounds__(NT) k_fused_all(Consts k, const float* __restr
     const GaugeOut go{a.gauges, a.ng, (a.gseries && s < a.gcap) ? a.gseries + (size_t)s * a.ng : nullptr};
     int m = 0;
     const double Amax = tile_step<TX, TY, NT>(k, W, Wn, H, dt, a.ctl->t, a.emax, tx * TX, ty * TY, nullptr, nullptr,
-                                              0x7fffffff, go, &m, tx * TX, ty * TY);
+                                              0x7fffffff, go, &m, tx * TX, ty * TY, a.ctl, s);
     general_tile_end<NT>(Amax, m, tc, a.ctl, tx, ty, s + 1);
 }
 
@@ -948,7 +957,8 @@ __global__ void __launch_bounds__(NT) k_batch(const BatchScen* __restrict__ sd, 
     const int cur = step & 1;
     const Consts k = d.k;
     const double Amax = tile_step<TX, TY, NT>(k, gmem(d.W[cur]), gmem(d.W[cur ^ 1]), gmem(d.H), ctl->dt_cur, ctl->t,
-                                              d.emax ? gmem(d.emax) : nullptr, tx * TX, ty * TY);
+                                              d.emax ? gmem(d.emax) : nullptr, tx * TX, ty * TY, nullptr, nullptr,
+                                              0x7fffffff, GaugeOut{nullptr, 0, nullptr}, nullptr, 0, 0, ctl, step);
     block_max_atomic<NT>(Amax, &ctl->maxA_next);
 }
 
@@ -974,7 +984,7 @@ __global__ void __launch_bounds__(NT) k_batch_bare(const BatchScen* __restrict__
     if (!tile_step_bare<TX, TY, NT>(k, gmem(d.W[cur]), gmem(d.W[cur ^ 1]), gmem(d.H), ctl->dt_cur, ctl->t,
                                     d.emax ? gmem(d.emax) : nullptr, tx * TX, ty * TY,
                                     gmem(d.cls) + (step & 1) * nct + tile, step, ctl->cls_from,
-                                    GaugeOut{nullptr, 0, nullptr}, Amax))
+                                    GaugeOut{nullptr, 0, nullptr}, Amax, ctl))
         return;
     block_max_atomic<NT>(Amax, &ctl->maxA_next);
 }
@@ -1044,7 +1054,7 @@ __global__ void __launch_bounds__(NT) k_batch_gen(const BatchScen* __restrict__ 
         const double A = tile_step<STX, TY, NT>(d.k, gmem(d.W[cur]), gmem(d.W[cur ^ 1]), gmem(d.H), ctl->dt_cur,
                                                 ctl->t, d.emax ? gmem(d.emax) : nullptr, cx0 + sub * STX, cy0,
                                                 nullptr, nullptr, cx0 + FTX, GaugeOut{nullptr, 0, nullptr}, &m, cx0,
-                                                cy0);
+                                                cy0, ctl, step);
         m = block_or(m);   // also orders this sub-tile's LDS reads before the next one's staging
         if (m && threadIdx.x < 64)
             mark_general(TileCls{gmem(d.cls), gmem(d.list), d.ctiles_x, d.ctiles_y}, ctl, tx, ty, step + 1, m);

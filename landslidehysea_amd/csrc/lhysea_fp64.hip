@@ -245,9 +245,11 @@ __global__ void __launch_bounds__(256) k_update(Consts k, const double* __restri
                 a.emax[2 * c + 1] = a.ctl->t;
             }
         }
-        cell_update(k, o, ac, dt, n);
+        unsigned hb = 0u;
+        cell_update(k, o, ac, dt, n, &hb);
         for (int f = 0; f < 6; f++) Wn[f * N + c] = n[f];
         for (int f = 0; f < 7; f++) acc[f * N + c] = 0.0;
+        raise_health(a.ctl, hb, a.ctl->step);
     }
     block_max_atomic<256>(A, &a.ctl->maxA_next);
 }
@@ -276,12 +278,13 @@ __device__ __forceinline__ void accumulate(double (&sA)[7][TY][CW], int r, int c
 // Returns this thread's max of the wave-speed sums of the cells it updated.
 // go: gauges of the new state.  smask (optional): OR of the class-tile halo
 // masks of this thread's cells that keep sediment, the class tile at
-// (cx0, cy0).
+// (cx0, cy0).  hc (optional): control block whose health word records the
+// cell updates' health bits (HEALTH_*) as those of step hs.
 template <int TX, int TY, int NT>
 __device__ __forceinline__ double tile_step64(const Consts& k, const double* __restrict__ W, double* __restrict__ Wn,
                                               const double* __restrict__ H, double dt, double tnow,
                                               double* __restrict__ emax, int i0, int j0, GaugeOut go, int* smask,
-                                              int cx0, int cy0)
+                                              int cx0, int cy0, Ctl* hc = nullptr, int hs = 0)
 {
     constexpr int CW = TX + 2;              // staged columns
     constexpr int RW = TY + 2;              // staged rows
@@ -414,6 +417,7 @@ __device__ __forceinline__ double tile_step64(const Consts& k, const double* __r
 
     // cell update of the tile's own cells
     double Amax = 0.0;
+    unsigned hb = 0u;
     for (int e = tid; e < TX * TY; e += NT) {
         const int r = e / TX, c = e % TX;
         const int gj = j0 + r, gi = i0 + c;
@@ -435,9 +439,9 @@ __device__ __forceinline__ double tile_step64(const Consts& k, const double* __r
         const bool bare = (o[3] == 0.0) && (o[4] == 0.0) && (o[5] == 0.0) && (ac[3] == 0.0) && (ac[4] == 0.0) &&
                           (ac[5] == 0.0);
         if (__all(bare))
-            cell_update1(k, o, ac, dt, n);
+            cell_update1(k, o, ac, dt, n, &hb);
         else
-            cell_update(k, o, ac, dt, n);
+            cell_update(k, o, ac, dt, n, &hb);
 #pragma unroll
         for (int f = 0; f < 6; f++) Wn[f * N + g] = n[f];
         if (go.out) {
@@ -447,6 +451,7 @@ __device__ __forceinline__ double tile_step64(const Consts& k, const double* __r
         }
         if (smask && (n[3] != 0.0 || n[4] != 0.0 || n[5] != 0.0)) *smask |= halo_mask_of<TX, TY>(gi - cx0, gj - cy0);
     }
+    raise_health(hc, hb, hs);
     if (go.out) gauges_of_tile<TX, TY, NT>(go, &sW[0][0][0], &sW[3][0][0], &sW[6][0][0], CW, nx, i0, j0, 0x7fffffff);
     return Amax;
 }
@@ -464,11 +469,14 @@ constexpr int GGRID = 1024;
 // From step skip_from + 2 on, Wn holds the state that step s - 2 wrote; a
 // tile that step took as sediment-free (entry other than s - 2) has zero
 // layer-2 planes there already, and only the layer-1 planes are written.
+// hc (optional): control block whose health word records the cell updates'
+// health bits as those of step s.
 template <int TX, int TY, int NT>
 __device__ __forceinline__ bool tile_step_bare64(const Consts& k, const double* __restrict__ W,
                                                  double* __restrict__ Wn, const double* __restrict__ H, double dt,
                                                  double tnow, double* __restrict__ emax, int i0, int j0,
-                                                 const int* cls, int s, int skip_from, GaugeOut go, double& Amax)
+                                                 const int* cls, int s, int skip_from, GaugeOut go, double& Amax,
+                                                 Ctl* hc = nullptr)
 {
     constexpr int CW = TX + 2, RW = TY + 2;
     __shared__ double sW[4][RW][CW];   // h1, q1x, q1y, H
@@ -600,6 +608,7 @@ __device__ __forceinline__ bool tile_step_bare64(const Consts& k, const double* 
 
     // cell update (layer 2 stays zero)
     Amax = 0.0;
+    unsigned hb = 0u;
     for (int e = tid; e < TX * TY; e += NT) {
         const int r = e / TX, c = e % TX;
         const int gj = j0 + r, gi = i0 + c;
@@ -616,7 +625,7 @@ __device__ __forceinline__ bool tile_step_bare64(const Consts& k, const double* 
                 emax[2 * g + 1] = tnow;
             }
         }
-        cell_update1(k, o, ac, dt, n);
+        cell_update1(k, o, ac, dt, n, &hb);
 #pragma unroll
         for (int f = 0; f < 3; f++) Wn[f * N + g] = n[f];
         if (zero2) {
@@ -625,6 +634,7 @@ __device__ __forceinline__ bool tile_step_bare64(const Consts& k, const double* 
         }
         if (go.out) sW[0][r + 1][c + 1] = n[0];   // the new h1 (read by no other thread now)
     }
+    raise_health(hc, hb, s);
     if (go.out) gauges_of_tile<TX, TY, NT>(go, &sW[0][0][0], nullptr, &sW[3][0][0], CW, nx, i0, j0, 0x7fffffff);
     return true;
 }
@@ -651,7 +661,7 @@ __global__ void __launch_bounds__(NT) k_fused_all(Consts k, const double* __rest
     const GaugeOut go{a.gauges, a.ng, (a.gseries && s < a.gcap) ? a.gseries + (size_t)s * a.ng : nullptr};
     int m = 0;
     const double Amax = tile_step64<TX, TY, NT>(k, W, Wn, H, step_dt(a.ctl, a.dt_seq), a.ctl->t, a.emax, tx * TX,
-                                                ty * TY, go, &m, tx * TX, ty * TY);
+                                                ty * TY, go, &m, tx * TX, ty * TY, a.ctl, s);
     general_tile_end<NT>(Amax, m, tc, a.ctl, tx, ty, s + 1);
 }
 
@@ -669,7 +679,7 @@ __global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const double* __res
     double Amax;
     if (!tile_step_bare64<TX, TY, NT>(k, W, Wn, H, step_dt(a.ctl, a.dt_seq), a.ctl->t, a.emax, tx * TX, ty * TY,
                                       tc.cls + (s & 1) * nb + b, s, a.wn_prev ? a.ctl->cls_from : 0x7fffffff, go,
-                                      Amax))
+                                      Amax, a.ctl))
         return;
     block_max_atomic<NT>(Amax, &a.ctl->maxA_next);
 }
@@ -700,7 +710,7 @@ __global__ void __launch_bounds__(NT) k_fused_gen(Consts k, const double* __rest
         const int tx = t % tc.tiles_x, ty = t / tc.tiles_x;
         int m = 0;
         Amax = fmax(Amax, tile_step64<TX, TY, NT>(k, W, Wn, H, dt, tnow, a.emax, tx * TX, ty * TY, go, &m, tx * TX,
-                                                  ty * TY));
+                                                  ty * TY, ctl, s));
         m = block_or(m);   // also orders this tile's LDS reads before the next one's staging
         if (m && threadIdx.x < 64) mark_general(tc, ctl, tx, ty, s + 1, m);
     }
@@ -711,6 +721,7 @@ __global__ void __launch_bounds__(NT) k_fused_gen(Consts k, const double* __rest
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < NT / 64; w++) v = fmax(v, red[w]);
+        v = fabs(v);   // sign bit clear also for a NaN from a corrupt state
         const unsigned long long old = atomicMax(&ctl->maxA_next, (unsigned long long)__double_as_longlong(v));
         // bit 63 of a non-negative double is clear: the increment is 1
         const unsigned done = atomicAdd(&ctl->blocks_done, 1u + (unsigned)(old >> 63));

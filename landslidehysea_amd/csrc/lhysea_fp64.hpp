@@ -403,14 +403,16 @@ __device__ __forceinline__ double edge_dt0(const Consts& k, const double* W0, co
 
 // Cell update: explicit step, filtroEstado, disImplicita, coulomb
 // (src/GPU/Volumen_kernel.cu:7-180, 231-284).  o[6] old state, acc[6]
-// flux sums; n[6] receives the new state.
+// flux sums; n[6] receives the new state.  hb (if given) collects the
+// health bits: a depth below -eps_h before the clip, a non-finite new state.
 __device__ __forceinline__ void cell_update(const Consts& k, const double* o, const double* acc, double dt,
-                                            double* n)
+                                            double* n, unsigned* hb = nullptr)
 {
     const double eps = k.eps_h, r = k.r, g = k.g;
     double val = dt / k.area;
     double n10 = o[0] + val * acc[0], n11 = o[1] + val * acc[1], n12 = o[2] + val * acc[2];
     double n20 = o[3] + val * acc[3], n21 = o[4] + val * acc[4], n22 = o[5] + val * acc[5];
+    if (hb && (n10 < -eps || n20 < -eps)) *hb |= HEALTH_NEGATIVE_DEPTH;
 
     // filtroEstado
     if (n10 < 0.0) n10 = 0.0;
@@ -499,6 +501,7 @@ __device__ __forceinline__ void cell_update(const Consts& k, const double* o, co
     }
     n[0] = n10; n[1] = n11; n[2] = n12;
     n[3] = n20; n[4] = n21; n[5] = n22;
+    if (hb && !(fabs(((n10 + n11) + (n12 + n20)) + (n21 + n22)) <= 1.7976931348623157e308)) *hb |= HEALTH_NONFINITE;
 }
 
 // cell_update() for a cell without sediment: o[3..5] and acc[3..5] are zero.
@@ -507,11 +510,12 @@ __device__ __forceinline__ void cell_update(const Consts& k, const double* o, co
 // case (for either friction law), so those steps are dropped; the layer-1
 // steps are cell_update()'s in its order and give the same values.
 __device__ __forceinline__ void cell_update1(const Consts& k, const double* o, const double* acc, double dt,
-                                             double* n)
+                                             double* n, unsigned* hb = nullptr)
 {
     const double eps = k.eps_h, g = k.g;
     double val = dt / k.area;
     double n10 = o[0] + val * acc[0], n11 = o[1] + val * acc[1], n12 = o[2] + val * acc[2];
+    if (hb && n10 < -eps) *hb |= HEALTH_NEGATIVE_DEPTH;
 
     // filtroEstado
     if (n10 < 0.0) n10 = 0.0;
@@ -542,6 +546,7 @@ __device__ __forceinline__ void cell_update1(const Consts& k, const double* o, c
     }
     n[0] = n10; n[1] = n11; n[2] = n12;
     n[3] = 0.0; n[4] = 0.0; n[5] = 0.0;
+    if (hb && !(fabs((n10 + n11) + n12) <= 1.7976931348623157e308)) *hb |= HEALTH_NONFINITE;
 }
 
 }  // namespace f64

@@ -13,7 +13,10 @@ namespace lhysea {
 
 // Shared by the C-ABI files (lhysea_capi.hip)
 Consts make_consts(const lhysea_params& p);
+// message of the failure being returned (also clears HIP's per-thread last error)
 void set_last_error(const std::string& msg);
+// LHYSEA_ERR_NUMERIC, the last error naming the health bits of c and their first step
+int numeric_fail(const Ctl& c);
 // argument checks of lhysea_create (0, or an LHYSEA_ERR_ code with the message set)
 int check_params(const lhysea_params& p);
 

@@ -64,7 +64,7 @@ int scratch_on_current_device(unsigned char** out)
 }
 
 // Control block at step 0 with time t and the dt of the first step (0: to be
-// computed); no class table yet.
+// computed); no class table yet, no health bit.
 __global__ void k_ctl_reset(Ctl* ctl, double t, double dt)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -76,6 +76,8 @@ __global__ void k_ctl_reset(Ctl* ctl, double t, double dt)
         ctl->maxA_last = 0.0;
         ctl->gcount[0] = ctl->gcount[1] = 0u;
         ctl->cls_from = 0x7fffffff;
+        ctl->health = 0u;
+        ctl->health_step = 0x7fffffff;
     }
 }
 
@@ -155,5 +157,5 @@ extern "C" int lhysea_step_device(const lhysea_params* p, const double* H, void*
     *t = c.t;
     *dt = c.dt_cur;
     *result_in_work = nsteps & 1;
-    return LHYSEA_OK;
+    return c.health ? numeric_fail(c) : LHYSEA_OK;
 }

@@ -23,6 +23,12 @@ FP64 = 0
 FP32C = 1
 KERNEL_FOURPASS = 0
 KERNEL_FUSED = 1
+# status codes and health bits of include/lhysea.h
+ERR_GPU_MEMORY = 1
+ERR_ARGUMENT = 3
+ERR_NUMERIC = 7
+HEALTH_NONFINITE = 1
+HEALTH_NEGATIVE_DEPTH = 2
 
 
 class NativeLibraryMissing(RuntimeError):
@@ -113,6 +119,7 @@ SIGNATURES = {
     "lhysea_run_until": (C.c_int, [C.c_void_p, C.c_double, C.c_int32, _ip]),
     "lhysea_sync": (C.c_int, [C.c_void_p]),
     "lhysea_get_time": (C.c_int, [C.c_void_p, _dp, _dp, _ip]),
+    "lhysea_health": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), _ip]),
     "lhysea_track_eta_max": (C.c_int, [C.c_void_p, C.c_int]),
     "lhysea_eta_max": (C.c_int, [C.c_void_p, _dp, _dp]),
     "lhysea_set_gauges": (C.c_int, [C.c_void_p, _ip, C.c_int32, C.c_int32]),
@@ -140,6 +147,7 @@ SIGNATURES = {
     "lhysea_batch_step_eager": (C.c_int, [C.c_void_p, C.c_int32]),
     "lhysea_batch_sync": (C.c_int, [C.c_void_p]),
     "lhysea_batch_get_time": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _ip]),
+    "lhysea_batch_health": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint32), _ip]),
     "lhysea_batch_track_eta_max": (C.c_int, [C.c_void_p, C.c_int32, C.c_int]),
     "lhysea_batch_eta_max": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
     "lhysea_batch_set_gauges": (C.c_int, [C.c_void_p, C.c_int32, _ip, C.c_int32, C.c_int32]),
@@ -253,6 +261,12 @@ class Simulation:
         t, d, n = C.c_double(), C.c_double(), C.c_int32()
         _check(self.lib.lhysea_get_time(self.h, C.byref(t), C.byref(d), C.byref(n)))
         return t.value, d.value, n.value
+
+    def health(self):
+        """(HEALTH_* bits raised since the last upload, first step that raised one or -1)."""
+        f, s = C.c_uint32(), C.c_int32()
+        _check(self.lib.lhysea_health(self.h, C.byref(f), C.byref(s)))
+        return f.value, s.value
 
     def track_eta_max(self, on: bool = True):
         _check(self.lib.lhysea_track_eta_max(self.h, 1 if on else 0))
@@ -413,6 +427,12 @@ class Batch:
         t, d, n = C.c_double(), C.c_double(), C.c_int32()
         _check(self.lib.lhysea_batch_get_time(self.h, sid, C.byref(t), C.byref(d), C.byref(n)))
         return t.value, d.value, n.value
+
+    def health(self, sid: int):
+        """Simulation.health() of scenario sid."""
+        f, s = C.c_uint32(), C.c_int32()
+        _check(self.lib.lhysea_batch_health(self.h, sid, C.byref(f), C.byref(s)))
+        return f.value, s.value
 
     def track_eta_max(self, sid: int, on: bool = True):
         _check(self.lib.lhysea_batch_track_eta_max(self.h, sid, 1 if on else 0))

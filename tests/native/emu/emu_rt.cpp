@@ -58,6 +58,14 @@ static void emu_fiber() {
     EmuBar& w = emu_wav[emu_cur / 64];
     w.expected--; emu_release(w);
 }
+void* emu_device_alloc(size_t n) {
+    static const size_t cap = [] {
+        const char* c = std::getenv("EMU_MEM_CAP");
+        return c ? (size_t)std::strtoull(c, nullptr, 10) : (size_t)-1;
+    }();
+    if (n > cap) return nullptr;
+    return calloc(n ? n : 1, 1);
+}
 static void emu_run(dim3 grid, dim3 block, const std::function<void()>& body) {
     gridDim = grid; blockDim = block;
     const unsigned nt = block.x * block.y * block.z;

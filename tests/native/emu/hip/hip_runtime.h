@@ -72,6 +72,10 @@ inline int __shfl_xor(int v, int off, int w = 64) { return (int)__shfl_xor((doub
 extern std::mutex emu_atomic_mu;
 inline int atomicOr(int* p, int v) {
     std::lock_guard<std::mutex> g(emu_atomic_mu); int o = *p; *p = o | v; return o; }
+inline unsigned atomicOr(unsigned* p, unsigned v) {
+    std::lock_guard<std::mutex> g(emu_atomic_mu); unsigned o = *p; *p = o | v; return o; }
+inline int atomicMin(int* p, int v) {
+    std::lock_guard<std::mutex> g(emu_atomic_mu); int o = *p; if (v < o) *p = v; return o; }
 inline int atomicMax(int* p, int v) {
     std::lock_guard<std::mutex> g(emu_atomic_mu); int o = *p; if (v > o) *p = v; return o; }
 inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v) {
@@ -117,7 +121,10 @@ inline hipError_t hipGetLastError() { return 0; }
 inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return 0; }
 inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) { p->major = 9; return 0; }
 inline hipError_t hipGetDevice(int* d) { *d = 0; return 0; }
-template <class T> inline hipError_t hipMalloc(T** p, size_t n) { *p = (T*)calloc(n ? n : 1, 1); return *p ? 0 : 2; }
+// device allocation: calloc, refused (out of memory) for a single request above
+// EMU_MEM_CAP bytes when that variable is set
+void* emu_device_alloc(size_t n);
+template <class T> inline hipError_t hipMalloc(T** p, size_t n) { *p = (T*)emu_device_alloc(n); return *p ? 0 : 2; }
 inline hipError_t hipFree(void* p) { free(p); return 0; }
 inline hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_cast<void*>(sym); return 0; }
 inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { std::memmove(d, s, n); return 0; }

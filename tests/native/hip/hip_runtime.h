@@ -21,3 +21,6 @@ inline float __expf(float x) { return std::exp(x); }
 inline float __powf(float a, float b) { return std::pow(a, b); }
 inline float __builtin_amdgcn_logf(float x) { return std::log2(x); }
 inline float __builtin_amdgcn_exp2f(float x) { return std::exp2(x); }
+// single-threaded host checks: plain read-modify-write
+inline unsigned atomicOr(unsigned* p, unsigned v) { unsigned o = *p; *p = o | v; return o; }
+inline int atomicMin(int* p, int v) { int o = *p; if (v < o) *p = v; return o; }
