@@ -255,12 +255,53 @@ int lhysea_batch_autotune(lhysea_batch *b, int32_t nsteps, double *ms_per_tile, 
 int lhysea_step_device(const lhysea_params *p, const double *H, void *W, void *W_work, int32_t nsteps,
                        double *t, double *dt, int32_t *result_in_work, void *stream);
 
-/* One-shot driver with shallowWater's contract: runs while t < t_end, writes
- * no files itself (output is the caller's, see L-HySEA.exe), returns the loop
- * wall time in *elapsed_s.  W is updated in place with the final state; eta
- * max value/time arrays may be NULL. */
+/* One-shot driver: runs while t < t_end and returns the final state in W
+ * (in place), eta max value/time (nx*ny each, may be NULL) and the loop wall
+ * time in *elapsed_s; writes no files (lhysea_shallow_water_output below is
+ * the form that writes shallowWater's output). */
 int lhysea_shallow_water(const lhysea_params *p, double *W, const double *H, double t_end,
                          double *eta_max, double *eta_max_time, double *elapsed_s, int32_t *steps_done);
+
+/* Output files of a run as shallowWater writes them (src/GPU/ShallowWater.cu:211-334,
+ * 462-474; layout of src/GPU/netcdf.cu:29-411): gauges NULL gives six NetCDF
+ * files <prefix>_{eta1,q1x,q1y,eta2,q2x,q2y}.nc (axes, bathymetry, one record
+ * per write, max_height at close); otherwise the text file
+ * <prefix>_eta_puntos.txt, one line per write (time, then eta1 at each gauge
+ * cell; -1 entries, gauges outside the grid, print -999).  Values in physical
+ * units: depths (H + Hmin) and lengths times p->H and p->L, discharges times
+ * Q, times times T (the reference's normalisation scales; all 1 when the run
+ * is not normalised).  Host code: no GPU is needed to write. */
+typedef struct lhysea_output_spec {
+    const char *prefix;        /* prefijo */
+    const char *grid_name;     /* nombre_bati: the grid_name attribute (may be NULL) */
+    double save_interval;      /* tiempo_guardar (internal units): < 0 never, 0 every step */
+    double xmin, ymin;         /* grid origin (internal units) */
+    double Hmin;               /* depth shift of H (<= 0) */
+    double Q, T;               /* discharge and time scales */
+    double sim_time;           /* tiempo_tot, the simulation_time attribute */
+    const int32_t *gauges;     /* NULL: NetCDF; else ngauges flat cell indices or -1 */
+    int32_t ngauges;
+} lhysea_output_spec;
+typedef struct lhysea_output lhysea_output;
+/* Creates the files (LHYSEA_ERR_IO if they cannot be created); H is the run's depth. */
+int lhysea_output_open(const lhysea_output_spec *spec, const lhysea_params *p, const double *H,
+                       lhysea_output **out);
+/* One record (NetCDF) or line (gauges) of the state W (6 planes, internal units) at time t. */
+int lhysea_output_write(lhysea_output *out, const double *W, double t);
+/* max_height from eta_max (h1 + h2 - H of lhysea_eta_max; NULL: not written),
+ * then the files are closed and out freed. */
+int lhysea_output_close(lhysea_output *out, const double *eta_max);
+
+/* shallowWater with its output (replaces src/GPU/ShallowWater.cu:125-484 as
+ * main.cxx calls it): the state is written at t = 0 and every save_interval
+ * while t < t_end, at most max_steps steps (INT32_MAX: no limit), then
+ * max_height; verbose prints the reference's per-step line
+ * ("Iteracion %3d, deltaT = %e seg, Tiempo = %g seg") and steps one step per
+ * call, otherwise the run advances to each save time in one call
+ * (lhysea_run_until).  W holds the final state on return. */
+int lhysea_shallow_water_output(const lhysea_params *p, double *W, const double *H, double t_end,
+                                int32_t max_steps, int32_t verbose, const lhysea_output_spec *spec,
+                                double *elapsed_s, int32_t *steps_done);
 
 #ifdef __cplusplus
 }

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblhysea_hip.so")
 SOURCES = ["lhysea_capi.hip", "lhysea_batch.hip", "lhysea_fp64.hip", "lhysea_fp32c.hip", "lhysea_adjoint.hip",
-           "lhysea_noalloc.hip"]
+           "lhysea_noalloc.hip", "host/lhysea_output.cpp", "host/lhysea_netcdf.cpp"]
 ARCH = os.environ.get("LHYSEA_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions stay off in every HIP translation unit.
@@ -26,7 +26,7 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-HOST_SOURCES = ["host/lhysea_main.cpp", "host/lhysea_problem.cpp", "host/lhysea_netcdf.cpp"]
+HOST_SOURCES = ["host/lhysea_main.cpp", "host/lhysea_problem.cpp"]
 EXE = os.path.join(HERE, "L-HySEA.exe")
 
 

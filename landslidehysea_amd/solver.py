@@ -105,6 +105,13 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 _lib = None
 
+
+class COutputSpec(C.Structure):
+    """Mirror of ``lhysea_output_spec`` (include/lhysea.h)."""
+    _fields_ = [("prefix", C.c_char_p), ("grid_name", C.c_char_p), ("save_interval", C.c_double),
+                ("xmin", C.c_double), ("ymin", C.c_double), ("Hmin", C.c_double), ("Q", C.c_double),
+                ("T", C.c_double), ("sim_time", C.c_double), ("gauges", _ip), ("ngauges", C.c_int32)]
+
 # every entry point of include/lhysea.h with its ctypes signature
 SIGNATURES = {
     "lhysea_device_probe": (C.c_int, []),
@@ -134,6 +141,11 @@ SIGNATURES = {
     "lhysea_adjoint": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, C.c_int32, _dp, _dp, _dp, _dp, _dp,
                                  _dp]),
     "lhysea_shallow_water": (C.c_int, [C.POINTER(CParams), _dp, _dp, C.c_double, _dp, _dp, _dp, _ip]),
+    "lhysea_output_open": (C.c_int, [C.POINTER(COutputSpec), C.POINTER(CParams), _dp, C.POINTER(C.c_void_p)]),
+    "lhysea_output_write": (C.c_int, [C.c_void_p, _dp, C.c_double]),
+    "lhysea_output_close": (C.c_int, [C.c_void_p, _dp]),
+    "lhysea_shallow_water_output": (C.c_int, [C.POINTER(CParams), _dp, _dp, C.c_double, C.c_int32, C.c_int32,
+                                              C.POINTER(COutputSpec), _dp, _ip]),
     "lhysea_step_device": (C.c_int, [C.POINTER(CParams), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, _dp, _dp,
                                      _ip, C.c_void_p]),
     "lhysea_batch_create": (C.c_int, [C.POINTER(CParams), C.c_int32, C.POINTER(C.c_void_p)]),
@@ -496,6 +508,54 @@ def shallow_water(params: Params, W: np.ndarray, H: np.ndarray, t_end: float):
     _check(lib.lhysea_shallow_water(C.byref(cp), _dptr(W), _dptr(H), t_end, _dptr(em), _dptr(et),
                                     C.byref(el), C.byref(n)))
     return W, em.reshape(params.ny, params.nx), et.reshape(params.ny, params.nx), el.value, n.value
+
+
+def output_spec(prefix: str, save_interval: float = -1.0, grid_name: str = "", xmin: float = 0.0,
+                ymin: float = 0.0, Hmin: float = 0.0, Q: float = 1.0, T: float = 1.0, sim_time: float = 0.0,
+                gauges=None):
+    """lhysea_output_spec; returns (spec, keep-alive) — the gauge array must outlive the calls."""
+    g = None if gauges is None else np.ascontiguousarray(gauges, dtype=np.int32)
+    spec = COutputSpec(prefix.encode(), grid_name.encode(), save_interval, xmin, ymin, Hmin, Q, T, sim_time,
+                       None if g is None else g.ctypes.data_as(_ip), 0 if g is None else len(g))
+    return spec, g
+
+
+class Output:
+    """Output files of a run (lhysea_output_open/write/close); host code, no GPU."""
+
+    def __init__(self, params: Params, H: np.ndarray, prefix: str, **spec_kw):
+        self.lib = load_library()
+        self._cp = params.to_c()
+        self._spec, self._keep = output_spec(prefix, **spec_kw)
+        self._H = np.ascontiguousarray(H, dtype=np.float64)
+        h = C.c_void_p()
+        _check(self.lib.lhysea_output_open(C.byref(self._spec), C.byref(self._cp), _dptr(self._H), C.byref(h)))
+        self.h = h
+
+    def write(self, W: np.ndarray, t: float):
+        _check(self.lib.lhysea_output_write(self.h, _dptr(np.ascontiguousarray(W, dtype=np.float64)), t))
+
+    def close(self, eta_max: np.ndarray | None = None):
+        if self.h:
+            em = None if eta_max is None else np.ascontiguousarray(eta_max, dtype=np.float64)
+            h, self.h = self.h, None
+            _check(self.lib.lhysea_output_close(h, None if em is None else _dptr(em)))
+
+
+def shallow_water_output(params: Params, W: np.ndarray, H: np.ndarray, t_end: float, prefix: str,
+                         max_steps: int = 2**31 - 1, verbose: bool = False, **spec_kw):
+    """lhysea_shallow_water_output: the run with shallowWater's output files;
+    returns (final W, seconds, steps)."""
+    lib = load_library()
+    cp = params.to_c()
+    spec, keep = output_spec(prefix, **spec_kw)
+    W = np.ascontiguousarray(W, dtype=np.float64).copy()
+    H = np.ascontiguousarray(H, dtype=np.float64)
+    el, n = C.c_double(), C.c_int32()
+    _check(lib.lhysea_shallow_water_output(C.byref(cp), _dptr(W), _dptr(H), t_end, max_steps, 1 if verbose else 0,
+                                           C.byref(spec), C.byref(el), C.byref(n)))
+    del keep
+    return W, el.value, n.value
 
 
 def step_device(params: Params, H_dev: int, W_dev: int, W_work_dev: int, nsteps: int, t: float = 0.0,

@@ -48,7 +48,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(ROOT, "landslidehysea_amd", "csrc")
 SOURCES = ["lhysea_capi.hip", "lhysea_batch.hip", "lhysea_fp64.hip", "lhysea_fp32c.hip", "lhysea_adjoint.hip",
-           "lhysea_noalloc.hip"]
+           "lhysea_noalloc.hip", "host/lhysea_output.cpp", "host/lhysea_netcdf.cpp"]
 
 
 def build_emulated(out: str):
