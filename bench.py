@@ -314,9 +314,53 @@ def summarize(dbs):
             pass
 
 
+def gaps(argv):
+    """Idle time between consecutive kernels in a rocprofv3 kernel trace
+    (--kernel-trace --output-format csv): `--gaps PATTERN LAST csv...` takes the
+    last LAST dispatches whose name contains PATTERN (the steady state of a
+    timed loop), in start order, and reports per kernel its count and mean
+    duration, and per transition (kernel -> next kernel) the count, mean,
+    median and maximum gap from one kernel's end to the next one's start."""
+    import csv
+    import statistics
+    pat, last, files = argv[0], int(argv[1]), argv[2:]
+    rows = []
+    for fn in files:
+        with open(fn, newline="") as f:
+            for r in csv.DictReader(f):
+                name = r["Kernel_Name"]
+                if pat in name:
+                    short = name.split("(")[0].split("<")[0].replace("void ", "").split("::")[-1]
+                    rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), short))
+    rows.sort()
+    rows = rows[-last:]
+    if len(rows) < 2:
+        print("fewer than two matching dispatches")
+        return
+    span_us = (rows[-1][1] - rows[0][0]) / 1e3
+    busy_us = sum(e - s for s, e, _ in rows) / 1e3
+    print(f"dispatches: {len(rows)}; first start to last end {span_us:.1f} us; kernels busy {busy_us:.1f} us "
+          f"({100 * busy_us / span_us:.2f}%); idle between kernels {span_us - busy_us:.1f} us")
+    per = {}
+    for s, e, n in rows:
+        per.setdefault(n, []).append((e - s) / 1e3)
+    print("kernel | dispatches | mean_us")
+    for n, d in per.items():
+        print(f"{n} | {len(d)} | {statistics.mean(d):.3f}")
+    tr = {}
+    for (s0, e0, n0), (s1, e1, n1) in zip(rows, rows[1:]):
+        tr.setdefault(f"{n0} -> {n1}", []).append((s1 - e0) / 1e3)
+    print("transition | count | mean_gap_us | median_gap_us | max_gap_us")
+    for k, g in tr.items():
+        print(f"{k} | {len(g)} | {statistics.mean(g):.3f} | {statistics.median(g):.3f} | {max(g):.3f}")
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize":
         summarize(sys.argv[2:])
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "--gaps":
+        gaps(sys.argv[2:])
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--copy-probe":
         copy_probe()
