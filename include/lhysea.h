@@ -168,7 +168,15 @@ int lhysea_time_steps(lhysea_sim *s, int32_t nsteps, double *total_ms, double *k
  * coefficient mu = |tan(angle1)| L/H and dL/dangle1 (may be NULL).
  * stats (6 doubles, may be NULL): forward ms, reverse ms, re-advanced steps,
  * reverse steps, snapshots, optimal re-advance count.  Afterwards the handle
- * must be uploaded again before further steps.
+ * must be uploaded again before further steps.  Both friction laws: under the
+ * Pouliquen law (friction_law 1) the coefficient depends on the state and on
+ * all four angles, grad_mu must be NULL (LHYSEA_ERR_ARGUMENT otherwise, the
+ * workspace kept) and grad_angle1 is dL/dangle1.
+ *
+ * lhysea_adjoint_backward_angles is the same sweep with the gradient of every
+ * friction angle: grad_angles[i] = dL/dangle(i+1), i = 0..3 (Coulomb: only
+ * angle1 enters, the other three are 0; Pouliquen: the four angles of
+ * src/GPU/Arista_kernel.cu:20-58).
  *
  * lhysea_adjoint is both for the gauge-misfit loss
  *     L = sum_n sum_g (eta_g^n - obs[n*ng + g])^2,   seeds 2 (eta - obs). */
@@ -176,6 +184,8 @@ int lhysea_adjoint_forward(lhysea_sim *s, int32_t nsteps, int32_t nsnap, const i
                            double *eta);
 int lhysea_adjoint_backward(lhysea_sim *s, const double *seeds, double *grad_W0, double *grad_mu,
                             double *grad_angle1, double *stats);
+int lhysea_adjoint_backward_angles(lhysea_sim *s, const double *seeds, double *grad_W0, double *grad_angles,
+                                   double *stats);
 int lhysea_adjoint(lhysea_sim *s, int32_t nsteps, int32_t nsnap, const int32_t *gauges, int32_t ng,
                    const double *obs, double *loss, double *grad_W0, double *grad_mu, double *grad_angle1,
                    double *stats);

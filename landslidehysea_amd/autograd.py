@@ -1,11 +1,13 @@
 """torch.autograd wrappers of the C-ABI adjoint pair (gradient checks only).
 
-``GaugeSeries.apply(h2_init, angle1, spec)`` runs ``lhysea_adjoint_forward``
+``GaugeSeries.apply(h2_init, angles, spec)`` runs ``lhysea_adjoint_forward``
 on the GPU and returns the gauge series eta (nsteps, ng); its backward feeds
 the incoming gradient dL/deta, for any loss L the caller builds from the
-series in torch, to ``lhysea_adjoint_backward`` and returns dL/dh2^0 (per
-cell) and dL/dangle1 (radians).  ``GaugeMisfit.apply`` is the one-call form
-for the gauge misfit L = sum (eta - obs)^2 (``lhysea_adjoint``).
+series in torch, to the reverse sweep and returns dL/dh2^0 (per cell) and
+the friction-angle gradient (radians) in the shape of ``angles``: a scalar
+angle1, or under the Pouliquen law the four angles (shape (4,)).
+``GaugeMisfit.apply`` is the one-call form for the gauge misfit
+L = sum (eta - obs)^2.
 
 With ``couple_water`` the initial water depth follows the sediment,
 h1^0 = max(0, H - Hsurf - h2^0) (water at rest over the slide), so dL/dh2^0
@@ -38,9 +40,11 @@ class MisfitSpec:
     couple_water: bool = False
 
 
-def _start(spec: MisfitSpec, h2_init: torch.Tensor, angle1: torch.Tensor):
-    """Parameters and initial state for sediment depth h2_init and angle1."""
-    prm = replace(spec.params, angle_deg=(float(angle1) * 180.0 / math.pi,))
+def _start(spec: MisfitSpec, h2_init: torch.Tensor, angles: torch.Tensor):
+    """Parameters and initial state for sediment depth h2_init and the friction
+    angle(s): a scalar angle1, or the four angles of the Pouliquen law."""
+    a = angles.detach().cpu().numpy().reshape(-1)
+    prm = replace(spec.params, angle_deg=tuple(float(x) * 180.0 / math.pi for x in a))
     W = spec.W0.copy()
     h2 = h2_init.detach().cpu().numpy().reshape(W.shape[1:])
     if spec.couple_water:
@@ -48,6 +52,13 @@ def _start(spec: MisfitSpec, h2_init: torch.Tensor, angle1: torch.Tensor):
         W[0] = np.maximum(0.0, surf - h2)
     W[3] = h2
     return prm, W, h2
+
+
+def _grad_angles(res: dict, angles: torch.Tensor) -> torch.Tensor:
+    """dL/dangle in the shape of the angle argument (radians)."""
+    if angles.numel() == 4:
+        return torch.tensor(np.asarray(res["grad_angles"]), dtype=torch.float64).reshape(angles.shape)
+    return torch.tensor(res["grad_angle1"], dtype=torch.float64).reshape(angles.shape)
 
 
 def _grad_h2(spec: MisfitSpec, h2: np.ndarray, g: np.ndarray, shape) -> torch.Tensor:
@@ -61,8 +72,8 @@ def _grad_h2(spec: MisfitSpec, h2: np.ndarray, g: np.ndarray, shape) -> torch.Te
 
 class GaugeSeries(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, h2_init: torch.Tensor, angle1: torch.Tensor, spec: MisfitSpec):
-        prm, W, h2 = _start(spec, h2_init, angle1)
+    def forward(ctx, h2_init: torch.Tensor, angles: torch.Tensor, spec: MisfitSpec):
+        prm, W, h2 = _start(spec, h2_init, angles)
         sim = L.Simulation(prm)
         try:
             sim.upload(W, spec.H)
@@ -71,7 +82,7 @@ class GaugeSeries(torch.autograd.Function):
             sim.close()
             raise
         # the handle keeps the checkpoints until the backward
-        ctx.sim, ctx.spec, ctx.h2, ctx.shape = sim, spec, h2, h2_init.shape
+        ctx.sim, ctx.spec, ctx.h2, ctx.shape, ctx.angles = sim, spec, h2, h2_init.shape, angles.detach()
         return torch.from_numpy(eta)
 
     @staticmethod
@@ -84,18 +95,17 @@ class GaugeSeries(torch.autograd.Function):
             ctx.sim = None
         ctx.stats = res
         g_h2 = _grad_h2(ctx.spec, ctx.h2, res["grad_W0"], ctx.shape)
-        return g_h2, torch.tensor(res["grad_angle1"], dtype=torch.float64), None
+        return g_h2, _grad_angles(res, ctx.angles), None
 
 
 class GaugeMisfit(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, h2_init: torch.Tensor, angle1: torch.Tensor, spec: MisfitSpec):
-        prm, W, h2 = _start(spec, h2_init, angle1)
+    def forward(ctx, h2_init: torch.Tensor, angles: torch.Tensor, spec: MisfitSpec):
+        prm, W, h2 = _start(spec, h2_init, angles)
         with L.Simulation(prm) as sim:
             sim.upload(W, spec.H)
             res = sim.adjoint(spec.nsteps, spec.nsnap, spec.gauges, spec.obs)
-        ctx.save_for_backward(_grad_h2(spec, h2, res["grad_W0"], h2_init.shape),
-                              torch.tensor(res["grad_angle1"], dtype=torch.float64))
+        ctx.save_for_backward(_grad_h2(spec, h2, res["grad_W0"], h2_init.shape), _grad_angles(res, angles))
         ctx.stats = res
         return torch.tensor(res["loss"], dtype=torch.float64)
 

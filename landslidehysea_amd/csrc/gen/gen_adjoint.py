@@ -72,11 +72,41 @@ def eig(f, flag, g, gp, eps, eps4):
     return sort_net([u - cg, uu - ci, uu + ci, u + cg])
 
 
-def build_edge():
+def friction(h1, u1, h2, u2, tans, LoverH, eps, eps4, r, g):
+    """Pouliquen friction coefficient (src/GPU/Arista_kernel.cu:20-58) in the
+    arithmetic of the FP32 step (friction32): the reduced-gravity Froude
+    number of the two layers, then the stop / start coefficients of the four
+    angles' tangents, in the three regimes of the Froude number."""
+    tan1, tan2, tan3, tan4 = tans
+    one = h1.prog.const(1.0)
+    x = 10.0 * h1 * rcp(eps)
+    rr = 1.0 - r * (1.0 - exp(-(x * x)))
+    igp = rcp(g * rr)
+    fr1 = u1 * u1 * sdes(h1, eps, eps4) * igp
+    fr2 = u2 * u2 * sdes(h2, eps, eps4) * igp
+    fr = sqrt(fr1 + fr2 + (1.0 - rr) * fr1 * fr2)
+    beta, L2, chi = 0.136, 8.0e-4, 1.0e-3
+    dstop, dstart = tan2 - tan1, tan4 - tan3
+    den = rcp(1.0 + h2 * (1.0 / L2))
+    fast = fr > beta
+    still = fabs(fr) < EPS
+    mid = (~fast) & (~still)
+    mf_fast = tan1 + dstop * rcp(1.0 + beta * h2 * rcp(select(fast, fr, one) * L2))
+    mf_still = tan3 + dstart * den
+    mustop, mustart = tan1 + dstop * den, tan3 + dstart * den
+    mf_mid = mustart + powc(select(mid, fr, one) * (1.0 / beta), chi) * (mustop - mustart)
+    return select(fast, mf_fast, select(still, mf_still, mf_mid)) * LoverH
+
+
+def build_edge(pouliquen=False):
     P = Prog("adj_edge")
     eps, eps4, g, gp, r, mu, dt, area, ln, n1x, n1y, intr = [
         P.param(n) for n in ("eps", "eps4", "g", "gp", "r", "mu", "dt", "area", "len", "n1x", "n1y", "intr")]
     H0, H1 = P.param("H0", "D"), P.param("H1", "D")
+    if pouliquen:
+        # the coefficient only decides the Coulomb stop flag (a frozen branch)
+        tans = [P.param(n) for n in ("tan1", "tan2", "tan3", "tan4")]
+        LoverH = P.param("LoverH")
     h10, h20, h11, h21 = [P.inp(n, "D") for n in ("h10", "h20", "h11", "h21")]
     q0 = [P.inp(f"q0{k}") for k in range(4)]
     q1 = [P.inp(f"q1{k}") for k in range(4)]
@@ -124,7 +154,8 @@ def build_edge():
     tp3 = g * h2ij * ((1.0 - r) * deta2 + r * deta1)
     x = 10.0 * h1ij * rcp(eps)
     fsc = 1.0 - r * (1.0 - exp(-(x * x)))
-    sc = fsc * mu * g * h2ij
+    muc = friction(h1ij, u1ij, h2ij, u2ij, tans, LoverH, eps, eps4, r, g) if pouliquen else mu
+    sc = fsc * muc * g * h2ij
     coul = fabs(h2ij * u2ij) < sc * dt
     t23 = select(coul, g * h2ij * r * deta1, tp3)
 
@@ -207,13 +238,19 @@ def build_edge():
     return P
 
 
-def build_cell():
+def build_cell(pouliquen=False):
     P = Prog("adj_cell")
     eps, eps4, r, g, mfc, mf0, mfs, dt, val, vmax1, vmax2 = [
         P.param(n) for n in ("eps", "eps4", "r", "g", "mfc", "mf0", "mfs", "dt", "val", "vmax1", "vmax2")]
+    if pouliquen:
+        LoverH = P.param("LoverH")
     o = [P.inp(f"o{k}") for k in range(6)]
     a = [P.inp(f"a{k}") for k in range(6)]
-    mu = P.inp("mu")
+    if pouliquen:
+        # the four angles' tangents: inputs, so that their adjoints come out
+        tans = [P.inp(n) for n in ("tan1", "tan2", "tan3", "tan4")]
+    else:
+        mu = P.inp("mu")
     z = P.const(0.0)
     one = P.const(1.0)
     so1, so2 = sdes(o[0], eps, eps4), sdes(o[3], eps, eps4)
@@ -270,7 +307,14 @@ def build_cell():
     # coulomb
     xx = 10.0 * n[0] * rcp(eps)
     fsc = 1.0 - r * (1.0 - exp(-(xx * xx)))
-    sc = fsc * mu * g * n[3] * dt
+    if pouliquen:
+        # the coefficient of the state after the implicit friction (cell_update())
+        u1 = sqrt(n[1] * n[1] + n[2] * n[2]) * sdes(n[0], eps, eps4)
+        u2 = sqrt(n[4] * n[4] + n[5] * n[5]) * sdes(n[3], eps, eps4)
+        muc = friction(n[0], u1, n[3], u2, tans, LoverH, eps, eps4, r, g)
+    else:
+        muc = mu
+    sc = fsc * muc * g * n[3] * dt
     nq = sqrt(n[4] * n[4] + n[5] * n[5])
     go = (nq + sc >= EPS) & (n[3] > 0.0) & (nq >= sc)
     fco = select(go, nq * rcp(select(go, nq + sc, one)), z)
@@ -305,6 +349,14 @@ def main():
     text.append("// sign of zero results.")
     text.append(emit(build_edge(), name="adj_edge_ns", zeros=EDGE_ZEROS, positive=EDGE_POS, nonneg=EDGE_NONNEG))
     text.append(emit(build_cell(), name="adj_cell_ns", zeros=CELL_ZEROS, positive=CELL_POS, nonneg=CELL_NONNEG))
+    # the Pouliquen law (friction_law 1): the edge's Coulomb flag and the cell's
+    # Coulomb scaling take the Pouliquen coefficient; the cell returns the
+    # adjoints of the four angles' tangents.  Edges and cells without sediment
+    # do not depend on the coefficient and keep the specialisations above.
+    text.append("// The Pouliquen law (friction_law 1).")
+    text.append(emit(build_edge(pouliquen=True), forward_only=True, name="adj_edge_pq"))
+    text.append(emit(build_edge(pouliquen=True), name="adj_edge_pq"))
+    text.append(emit(build_cell(pouliquen=True), name="adj_cell_pq"))
     with open(OUT, "w") as f:
         f.write("\n".join(text))
     print(OUT)

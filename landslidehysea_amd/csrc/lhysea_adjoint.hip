@@ -47,7 +47,8 @@ struct Work {
     F* ACC;        // [6][N] flux sums
     F* AB;         // [6][N] adjoints of the flux sums
     double* LAM;   // [6][N] state adjoint (h1,q1x,q1y,h2,q2x,q2y)
-    double* gmu;   // d/dmu, one slot per workgroup
+    double* gmu;   // d/dmu, one slot per workgroup (Pouliquen: d/dtan(angle i) at gmu[i * gstride + slot])
+    size_t gstride;
 };
 
 // Edge geometry of edge e in class cls (the reference's class layout,
@@ -139,6 +140,7 @@ __device__ __forceinline__ void edge_inputs(const SV<F>& sv, const double* H, co
 template <typename F>
 struct EC {
     F eps, eps4, g, gp, r, mu, dt, area;
+    F tan1, tan2, tan3, tan4, LoverH;   // Pouliquen law
 };
 
 template <typename F>
@@ -153,11 +155,17 @@ __device__ __forceinline__ EC<F> edge_consts(const Consts& k, double dt)
     c.mu = (F)k.mu;
     c.dt = (F)dt;
     c.area = (F)k.area;
+    c.tan1 = (F)k.tan1;
+    c.tan2 = (F)k.tan2;
+    c.tan3 = (F)k.tan3;
+    c.tan4 = (F)k.tan4;
+    c.LoverH = (F)k.LoverH;
     return c;
 }
 
 // 1. forward recomputation of one class, keeping the partial sums it reads
-template <typename F>
+// PQ: the Pouliquen law (friction_law 1) in this and the other adjoint kernels
+template <typename F, bool PQ>
 __global__ void __launch_bounds__(256) k_recompute(Consts k, SV<F> sv, const double* H, Work<F> w,
                                                    const double* dt_hist, int step, int cls)
 {
@@ -180,10 +188,18 @@ __global__ void __launch_bounds__(256) k_recompute(Consts k, SV<F> sv, const dou
     EdgeIn<F> in;
     edge_inputs(sv, H, r, in);
     F fm[6], fp[6];
-    adj_edge_fwd<F>((F)c.eps, (F)c.eps4, (F)c.g, (F)c.gp, (F)c.r, (F)c.mu, (F)c.dt, (F)c.area, (F)r.len, (F)r.n1x,
-                    (F)r.n1y, r.interior ? F(1) : F(0), in.H0, in.H1, in.h10, in.h20, in.h11, in.h21, in.q0[0],
-                    in.q0[1], in.q0[2], in.q0[3], in.q1[0], in.q1[1], in.q1[2], in.q1[3], a0h1, a0h2, a1h1, a1h2,
-                    fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fp[0], fp[1], fp[2], fp[3], fp[4], fp[5]);
+    if constexpr (PQ)
+        adj_edge_pq_fwd<F>((F)c.eps, (F)c.eps4, (F)c.g, (F)c.gp, (F)c.r, (F)c.mu, (F)c.dt, (F)c.area, (F)r.len,
+                           (F)r.n1x, (F)r.n1y, r.interior ? F(1) : F(0), in.H0, in.H1, c.tan1, c.tan2, c.tan3, c.tan4,
+                           c.LoverH, in.h10, in.h20, in.h11, in.h21, in.q0[0], in.q0[1], in.q0[2], in.q0[3], in.q1[0],
+                           in.q1[1], in.q1[2], in.q1[3], a0h1, a0h2, a1h1, a1h2, fm[0], fm[1], fm[2], fm[3], fm[4],
+                           fm[5], fp[0], fp[1], fp[2], fp[3], fp[4], fp[5]);
+    else
+        adj_edge_fwd<F>((F)c.eps, (F)c.eps4, (F)c.g, (F)c.gp, (F)c.r, (F)c.mu, (F)c.dt, (F)c.area, (F)r.len,
+                        (F)r.n1x, (F)r.n1y, r.interior ? F(1) : F(0), in.H0, in.H1, in.h10, in.h20, in.h11, in.h21,
+                        in.q0[0], in.q0[1], in.q0[2], in.q0[3], in.q1[0], in.q1[1], in.q1[2], in.q1[3], a0h1, a0h2,
+                        a1h1, a1h2, fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fp[0], fp[1], fp[2], fp[3], fp[4],
+                        fp[5]);
     for (int f = 0; f < 6; f++) w.ACC[f * N + r.c0] -= fm[f];
     if (r.interior)
         for (int f = 0; f < 6; f++) w.ACC[f * N + r.c1] -= fp[f];
@@ -202,12 +218,14 @@ __device__ __forceinline__ double block_sum256(double v)
 }
 
 // 2. cell adjoint: lambda^{n+1} -> flux-sum adjoints, direct lambda^n, d/dmu
-template <typename F>
+//    (Pouliquen: d/dtan of the four angles)
+template <typename F, bool PQ>
 __global__ void __launch_bounds__(256) k_cell_adj(Consts k, SV<F> sv, Work<F> w, const double* dt_hist, int step)
 {
+    constexpr int NG = PQ ? 4 : 1;
     const size_t N = sv.N;
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double gm = 0.0;
+    double gm[NG] = {};
     if (c < N) {
         const double dt = dt_hist[step];
         const F eps = (F)k.eps_h;
@@ -226,25 +244,39 @@ __global__ void __launch_bounds__(256) k_cell_adj(Consts k, SV<F> sv, Work<F> w,
         // specialisation (same values and adjoints)
         const bool bare = (o[3] == F(0)) && (o[4] == F(0)) && (o[5] == F(0)) && (a[3] == F(0)) && (a[4] == F(0)) &&
                           (a[5] == F(0));
-        if (__all(bare))
+        // without sediment the Coulomb scaling multiplies zeros: either law
+        // takes the specialisation, whose coefficient adjoint is zero
+        if (__all(bare)) {
             adj_cell_ns<F>(LH_CELL_ARGS);
-        else
+        } else if constexpr (PQ) {
+            F gt[4];
+            adj_cell_pq<F>(eps, eps * eps * eps * eps, (F)k.r, (F)k.g, (F)k.mfc, (F)k.mf0, (F)k.mfs, (F)dt,
+                           (F)(dt / k.area), (F)k.vmax1, (F)k.vmax2, (F)k.LoverH, o[0], o[1], o[2], o[3], o[4], o[5],
+                           a[0], a[1], a[2], a[3], a[4], a[5], (F)k.tan1, (F)k.tan2, (F)k.tan3, (F)k.tan4, n[0], n[1],
+                           n[2], n[3], n[4], n[5], gn[0], gn[1], gn[2], gn[3], gn[4], gn[5], go[0], go[1], go[2],
+                           go[3], go[4], go[5], ga[0], ga[1], ga[2], ga[3], ga[4], ga[5], gt[0], gt[1], gt[2], gt[3]);
+            for (int i = 0; i < NG; i++) gm[i] = (double)gt[i];
+        } else {
             adj_cell<F>(LH_CELL_ARGS);
+            gm[0] = (double)gmu;
+        }
 #undef LH_CELL_ARGS
         for (int f = 0; f < 6; f++) {
             w.LAM[f * N + c] = (double)go[f];
             w.AB[f * N + c] = ga[f];
         }
-        gm = (double)gmu;
     }
-    // one d/dmu slot per workgroup, summed in slot order at the end of the
-    // sweep: the result does not depend on the workgroups' timing
-    const double s = block_sum256(gm);
-    if (threadIdx.x == 0 && s != 0.0) w.gmu[blockIdx.x] += s;
+    // one slot per workgroup (and component), summed in slot order at the end
+    // of the sweep: the result does not depend on the workgroups' timing
+    for (int i = 0; i < NG; i++) {
+        const double s = block_sum256(gm[i]);
+        if (threadIdx.x == 0 && s != 0.0) w.gmu[i * w.gstride + blockIdx.x] += s;
+        __syncthreads();   // block_sum256's LDS slots are reused
+    }
 }
 
 // 3. edge adjoints of one class
-template <typename F>
+template <typename F, bool PQ>
 __global__ void __launch_bounds__(256) k_edge_adj(Consts k, SV<F> sv, const double* H, Work<F> w,
                                                   const double* dt_hist, int step, int cls)
 {
@@ -283,10 +315,20 @@ __global__ void __launch_bounds__(256) k_edge_adj(Consts k, SV<F> sv, const doub
     // partial sums) takes the specialisation (same values and adjoints)
     const bool bare = (in.h20 == 0.0) && (in.h21 == 0.0) && (in.q0[2] == F(0)) && (in.q0[3] == F(0)) &&
                       (in.q1[2] == F(0)) && (in.q1[3] == F(0)) && (a0h2 == F(0)) && (a1h2 == F(0));
-    if (__all(bare))
+    // without sediment the Coulomb flag is false under either law
+    if (__all(bare)) {
         adj_edge_ns<F>(LH_EDGE_ARGS);
-    else
+    } else if constexpr (PQ) {
+        adj_edge_pq<F>((F)c.eps, (F)c.eps4, (F)c.g, (F)c.gp, (F)c.r, (F)c.mu, (F)c.dt, (F)c.area, (F)r.len, (F)r.n1x,
+                       (F)r.n1y, r.interior ? F(1) : F(0), in.H0, in.H1, c.tan1, c.tan2, c.tan3, c.tan4, c.LoverH,
+                       in.h10, in.h20, in.h11, in.h21, in.q0[0], in.q0[1], in.q0[2], in.q0[3], in.q1[0], in.q1[1],
+                       in.q1[2], in.q1[3], a0h1, a0h2, a1h1, a1h2, fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fp[0],
+                       fp[1], fp[2], fp[3], fp[4], fp[5], gfm[0], gfm[1], gfm[2], gfm[3], gfm[4], gfm[5], gfp[0],
+                       gfp[1], gfp[2], gfp[3], gfp[4], gfp[5], gh10, gh20, gh11, gh21, gq0[0], gq0[1], gq0[2], gq0[3],
+                       gq1[0], gq1[1], gq1[2], gq1[3], ga0h1, ga0h2, ga1h1, ga1h2);
+    } else {
         adj_edge<F>(LH_EDGE_ARGS);
+    }
 #undef LH_EDGE_ARGS
     double* L = w.LAM;
     if (r.interior) {
@@ -359,23 +401,26 @@ __global__ void k_set_step(Ctl* ctl, int step)
 // inside the box (LDS updates; within a class no two edges share a cell), and
 // writes lambda^n of its own cells.  lambda^{n+1} is read from one buffer and
 // lambda^n written to another.
-template <int TX, int TY, int NT>
+// PQ: the Pouliquen law; the workgroup's d/dtan of the four angles go to
+// gmu[i * gstride + blockIdx.x] (Coulomb: d/dmu to gmu[blockIdx.x]).
+template <int TX, int TY, int NT, bool PQ>
 __global__ void __launch_bounds__(NT) k_adj_fused(Consts k, const float* __restrict__ W,
                                                   const double* __restrict__ H, const float* __restrict__ P,
                                                   const float* __restrict__ ACC, const double* __restrict__ LIN,
-                                                  double* __restrict__ LOUT, double* gmu,
+                                                  double* __restrict__ LOUT, double* gmu, size_t gstride,
                                                   const double* __restrict__ dt_hist, int step, int tiles_x,
                                                   int tiles_y, const int* __restrict__ gauges, int ng,
                                                   const double* __restrict__ Gk)
 {
     static_assert(TX % 2 == 0 && TY % 2 == 0, "tile origins must stay even");
+    constexpr int NG = PQ ? 4 : 1;
     constexpr int HW = 2, CW = TX + 2 * HW, RW = TY + 2 * HW;
     __shared__ double sh1[RW][CW], sh2[RW][CW], sH[RW][CW];
     __shared__ float sq[4][RW][CW];
     __shared__ float sP[3][2][RW][CW];
     __shared__ float sAB[6][RW][CW];
     __shared__ double sL[6][TY][TX];
-    __shared__ double red[NT / 64];
+    __shared__ double red[NG][NT / 64];
 
     const int nx = k.nx, ny = k.ny;
     const size_t N = (size_t)nx * ny;
@@ -394,7 +439,7 @@ __global__ void __launch_bounds__(NT) k_adj_fused(Consts k, const float* __restr
     const float eps = (float)k.eps_h;
 
     // 1. stage the box; cell adjoint of every box cell
-    double gm = 0.0;
+    double gm[NG] = {};
     for (int e = tid; e < RW * CW; e += NT) {
         const int r = e / CW, cc = e % CW;
         const int gj = bj + r, gi = bi + cc;
@@ -418,7 +463,7 @@ __global__ void __launch_bounds__(NT) k_adj_fused(Consts k, const float* __restr
             sP[m][0][r][cc] = P[(size_t)m * 2 * N + g];
             sP[m][1][r][cc] = P[(size_t)m * 2 * N + N + g];
         }
-        float a[6], gn[6], n[6], go[6], ga[6], gmuc;
+        float a[6], gn[6], n[6], go[6], ga[6], gmuc, gt[NG];
 #pragma unroll
         for (int f = 0; f < 6; f++) {
             a[f] = ACC[f * N + g];
@@ -431,10 +476,23 @@ __global__ void __launch_bounds__(NT) k_adj_fused(Consts k, const float* __restr
         go[0], go[1], go[2], go[3], go[4], go[5], ga[0], ga[1], ga[2], ga[3], ga[4], ga[5], gmuc
         const bool bare = (o[3] == 0.0f) && (o[4] == 0.0f) && (o[5] == 0.0f) && (a[3] == 0.0f) && (a[4] == 0.0f) &&
                           (a[5] == 0.0f);
-        if (__all(bare))
+        // without sediment the Coulomb scaling multiplies zeros under either law
+        if (__all(bare)) {
             adj_cell_ns<float>(LH_CELL_ARGS);
-        else
+            gt[0] = PQ ? 0.0f : gmuc;   // the coefficient adjoint is zero without sediment
+#pragma unroll
+            for (int i = 1; i < NG; i++) gt[i] = 0.0f;
+        } else if constexpr (PQ) {
+            adj_cell_pq<float>(eps, eps * eps * eps * eps, (float)k.r, (float)k.g, (float)k.mfc, (float)k.mf0,
+                               (float)k.mfs, (float)dt, (float)(dt / k.area), (float)k.vmax1, (float)k.vmax2,
+                               c.LoverH, o[0], o[1], o[2], o[3], o[4], o[5], a[0], a[1], a[2], a[3], a[4], a[5],
+                               c.tan1, c.tan2, c.tan3, c.tan4, n[0], n[1], n[2], n[3], n[4], n[5], gn[0], gn[1],
+                               gn[2], gn[3], gn[4], gn[5], go[0], go[1], go[2], go[3], go[4], go[5], ga[0], ga[1],
+                               ga[2], ga[3], ga[4], ga[5], gt[0], gt[1], gt[2], gt[3]);
+        } else {
             adj_cell<float>(LH_CELL_ARGS);
+            gt[0] = gmuc;
+        }
 #undef LH_CELL_ARGS
 #pragma unroll
         for (int f = 0; f < 6; f++) sAB[f][r][cc] = ga[f];
@@ -442,7 +500,8 @@ __global__ void __launch_bounds__(NT) k_adj_fused(Consts k, const float* __restr
         if (tr >= 0 && tr < TY && tc >= 0 && tc < TX) {
 #pragma unroll
             for (int f = 0; f < 6; f++) sL[f][tr][tc] = (double)go[f];
-            gm += (double)gmuc;
+#pragma unroll
+            for (int i = 0; i < NG; i++) gm[i] += (double)gt[i];
         }
     }
     __syncthreads();
@@ -496,10 +555,20 @@ __global__ void __launch_bounds__(NT) k_adj_fused(Consts k, const float* __restr
         gh21, gq0[0], gq0[1], gq0[2], gq0[3], gq1[0], gq1[1], gq1[2], gq1[3], ga0h1, ga0h2, ga1h1, ga1h2
         const bool bare = (in.h20 == 0.0) && (in.h21 == 0.0) && (in.q0[2] == 0.0f) && (in.q0[3] == 0.0f) &&
                           (in.q1[2] == 0.0f) && (in.q1[3] == 0.0f) && (a0h2 == 0.0f) && (a1h2 == 0.0f);
-        if (__all(bare))
+        // without sediment the Coulomb flag is false under either law
+        if (__all(bare)) {
             adj_edge_ns<float>(LH_EDGE_ARGS);
-        else
+        } else if constexpr (PQ) {
+            adj_edge_pq<float>(c.eps, c.eps4, c.g, c.gp, c.r, c.mu, c.dt, c.area, len, n1x, n1y,
+                               interior ? 1.0f : 0.0f, in.H0, in.H1, c.tan1, c.tan2, c.tan3, c.tan4, c.LoverH, in.h10,
+                               in.h20, in.h11, in.h21, in.q0[0], in.q0[1], in.q0[2], in.q0[3], in.q1[0], in.q1[1],
+                               in.q1[2], in.q1[3], a0h1, a0h2, a1h1, a1h2, fm[0], fm[1], fm[2], fm[3], fm[4], fm[5],
+                               fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], gfm[0], gfm[1], gfm[2], gfm[3], gfm[4],
+                               gfm[5], gfp[0], gfp[1], gfp[2], gfp[3], gfp[4], gfp[5], gh10, gh20, gh11, gh21, gq0[0],
+                               gq0[1], gq0[2], gq0[3], gq1[0], gq1[1], gq1[2], gq1[3], ga0h1, ga0h2, ga1h1, ga1h2);
+        } else {
             adj_edge<float>(LH_EDGE_ARGS);
+        }
 #undef LH_EDGE_ARGS
         // lambda^n of the tile's own cells
         auto add_lambda = [&](int r, int cc, double dh1, double dh2, const double* dq) {
@@ -616,13 +685,19 @@ __global__ void __launch_bounds__(NT) k_adj_fused(Consts k, const float* __restr
 #pragma unroll
         for (int f = 0; f < 6; f++) LOUT[f * N + g] = sL[f][tr][tc];
     }
-    for (int off = 32; off > 0; off >>= 1) gm += __shfl_xor(gm, off, 64);
-    if ((tid & 63) == 0) red[tid >> 6] = gm;
+#pragma unroll
+    for (int i = 0; i < NG; i++) {
+        double v = gm[i];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((tid & 63) == 0) red[i][tid >> 6] = v;
+    }
     __syncthreads();
     if (tid == 0) {
-        double s = 0.0;
-        for (int w = 0; w < NT / 64; w++) s += red[w];
-        if (s != 0.0) gmu[blockIdx.x] += s;   // this workgroup's d/dmu slot
+        for (int i = 0; i < NG; i++) {
+            double s = 0.0;
+            for (int w = 0; w < NT / 64; w++) s += red[i][w];
+            if (s != 0.0) gmu[i * gstride + blockIdx.x] += s;   // this workgroup's slot
+        }
     }
 }
 
@@ -719,13 +794,13 @@ hipError_t run_treeverse(AdjointHooks& h, int S, long N) { return treeverse(h, S
 constexpr int kAdjTX = 60, kAdjTY = 12;
 
 // launch wrappers for one reverse step (four-pass structure)
-template <typename F>
+template <typename F, bool PQ>
 hipError_t reverse_step_t(const Consts& k, const F* W, const double* H, F* P, F* ACC, F* AB, double* LAM, double* gmu,
-                          const double* dt_hist, int step, hipStream_t s)
+                          size_t gstride, const double* dt_hist, int step, hipStream_t s)
 {
     const size_t N = (size_t)k.nx * k.ny;
     adj::SV<F> sv{W, N};
-    adj::Work<F> w{P, ACC, AB, LAM, gmu};
+    adj::Work<F> w{P, ACC, AB, LAM, gmu, gstride};
     long ne[4];
     for (int cls = 0; cls < 4; cls++)
         ne[cls] = (cls < 2) ? (long)((k.ny - cls) / 2 + 1) * k.nx : (long)((k.nx - (cls - 2)) / 2 + 1) * k.ny;
@@ -737,41 +812,54 @@ hipError_t reverse_step_t(const Consts& k, const F* W, const double* H, F* P, F*
         hipError_t e = hipMemsetAsync(ACC, 0, 6 * N * sizeof(F), s);
         if (e != hipSuccess) return e;
         for (int cls = 0; cls < 4; cls++)
-            hipLaunchKernelGGL(adj::k_recompute<F>, dim3((unsigned)((ne[cls] + 255) / 256)), dim3(256), 0, s, k, sv,
-                               H, w, dt_hist, step, cls);
+            hipLaunchKernelGGL((adj::k_recompute<F, PQ>), dim3((unsigned)((ne[cls] + 255) / 256)), dim3(256), 0, s, k,
+                               sv, H, w, dt_hist, step, cls);
     }
-    hipLaunchKernelGGL(adj::k_cell_adj<F>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, k, sv, w, dt_hist,
+    hipLaunchKernelGGL((adj::k_cell_adj<F, PQ>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, k, sv, w, dt_hist,
                        step);
     for (int cls = 3; cls >= 0; cls--)
-        hipLaunchKernelGGL(adj::k_edge_adj<F>, dim3((unsigned)((ne[cls] + 255) / 256)), dim3(256), 0, s, k, sv, H,
-                           w, dt_hist, step, cls);
+        hipLaunchKernelGGL((adj::k_edge_adj<F, PQ>), dim3((unsigned)((ne[cls] + 255) / 256)), dim3(256), 0, s, k, sv,
+                           H, w, dt_hist, step, cls);
     return hipGetLastError();
 }
 
 hipError_t adj_reverse_step_f32(const Consts& k, const float* W, const double* H, float* P, float* ACC, float* AB,
-                                double* LAM, double* gmu, const double* dt_hist, int step, hipStream_t s)
+                                double* LAM, double* gmu, size_t gstride, const double* dt_hist, int step,
+                                hipStream_t s)
 {
-    return reverse_step_t<float>(k, W, H, P, ACC, AB, LAM, gmu, dt_hist, step, s);
+    if (k.friction_law == 1)
+        return reverse_step_t<float, true>(k, W, H, P, ACC, AB, LAM, gmu, gstride, dt_hist, step, s);
+    return reverse_step_t<float, false>(k, W, H, P, ACC, AB, LAM, gmu, gstride, dt_hist, step, s);
 }
 
 hipError_t adj_reverse_step_f32_fused(const Consts& k, const float* W, const double* H, float* P, float* ACC,
-                                      const double* LAMin, double* LAMout, double* gmu, const double* dt_hist,
-                                      int step, const int* gauges, int ng, const double* Gk, hipStream_t s)
+                                      const double* LAMin, double* LAMout, double* gmu, size_t gstride,
+                                      const double* dt_hist, int step, const int* gauges, int ng, const double* Gk,
+                                      hipStream_t s)
 {
     constexpr int TX = kAdjTX, TY = kAdjTY, NT = 512;
     hipError_t e = f32c_recompute(k, W, H, dt_hist, step, P, ACC, s);
     if (e != hipSuccess) return e;
     const int tiles_x = (k.nx + TX - 1) / TX, tiles_y = (k.ny + TY - 1) / TY;
-    hipLaunchKernelGGL((adj::k_adj_fused<TX, TY, NT>), dim3((unsigned)(tiles_x * tiles_y)), dim3(NT), 0, s, k, W, H,
-                       P, ACC, LAMin, LAMout, gmu, dt_hist, step, tiles_x, tiles_y, gauges, ng > 0 ? ng : 0,
-                       ng > 0 ? Gk : nullptr);
+    const dim3 grid((unsigned)(tiles_x * tiles_y));
+    if (k.friction_law == 1)
+        hipLaunchKernelGGL((adj::k_adj_fused<TX, TY, NT, true>), grid, dim3(NT), 0, s, k, W, H, P, ACC, LAMin, LAMout,
+                           gmu, gstride, dt_hist, step, tiles_x, tiles_y, gauges, ng > 0 ? ng : 0,
+                           ng > 0 ? Gk : nullptr);
+    else
+        hipLaunchKernelGGL((adj::k_adj_fused<TX, TY, NT, false>), grid, dim3(NT), 0, s, k, W, H, P, ACC, LAMin, LAMout,
+                           gmu, gstride, dt_hist, step, tiles_x, tiles_y, gauges, ng > 0 ? ng : 0,
+                           ng > 0 ? Gk : nullptr);
     return hipGetLastError();
 }
 
 hipError_t adj_reverse_step_f64(const Consts& k, const double* W, const double* H, double* P, double* ACC, double* AB,
-                                double* LAM, double* gmu, const double* dt_hist, int step, hipStream_t s)
+                                double* LAM, double* gmu, size_t gstride, const double* dt_hist, int step,
+                                hipStream_t s)
 {
-    return reverse_step_t<double>(k, W, H, P, ACC, AB, LAM, gmu, dt_hist, step, s);
+    if (k.friction_law == 1)
+        return reverse_step_t<double, true>(k, W, H, P, ACC, AB, LAM, gmu, gstride, dt_hist, step, s);
+    return reverse_step_t<double, false>(k, W, H, P, ACC, AB, LAM, gmu, gstride, dt_hist, step, s);
 }
 
 size_t adj_gmu_slots(const Consts& k)

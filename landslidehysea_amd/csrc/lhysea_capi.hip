@@ -683,8 +683,9 @@ struct lhysea_adj_work {
     void* AB = nullptr;    // [6][N]
     double* LAM = nullptr;   // [6][N] state adjoint lambda of the current step
     double* LAM2 = nullptr;  // [6][N] second lambda buffer of the fused FP32 reverse step
-    double* gmu = nullptr;   // d/dmu slots, one per workgroup
+    double* gmu = nullptr;   // friction adjoint slots: ngmu per component (Coulomb 1, Pouliquen 4)
     size_t ngmu = 0;
+    int ngcomp = 1;
     const char* W = nullptr; // stored state of the reverse step (all states kept), else null
     int* gauges = nullptr;
     double* gser = nullptr;  // gauge series of the forward pass
@@ -769,12 +770,13 @@ hipError_t hook_reverse(void* c, int k)
     const double* Gk = (k >= 1 && a->ng > 0) ? a->G + (size_t)(k - 1) * a->ng : nullptr;
     if (s->p.precision == LHYSEA_FP64) {
         e = adj_reverse_step_f64(s->k, (const double*)W, s->b64.H, (double*)a->P, (double*)a->ACC, (double*)a->AB,
-                                 a->LAM, a->gmu, s->dt_hist, k, s->stream);
+                                 a->LAM, a->gmu, a->ngmu, s->dt_hist, k, s->stream);
         if (e == hipSuccess && Gk) e = adj_inject(a->LAM, s->N, a->gauges, a->ng, Gk, s->stream);
     } else {
         // lambda^{n+1} in LAM, lambda^n (with the seeds) into LAM2, then swap
         e = adj_reverse_step_f32_fused(s->k, (const float*)W, s->b32.H, (float*)a->P, (float*)a->ACC, a->LAM,
-                                       a->LAM2, a->gmu, s->dt_hist, k, a->gauges, Gk ? a->ng : 0, Gk, s->stream);
+                                       a->LAM2, a->gmu, a->ngmu, s->dt_hist, k, a->gauges, Gk ? a->ng : 0, Gk,
+                                       s->stream);
         std::swap(a->LAM, a->LAM2);
     }
     return e;
@@ -801,7 +803,6 @@ int lhysea_adjoint_forward(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const i
     if (!s || nsteps < 1 || nsnap < 1 || ng < 0 || (ng > 0 && !gauges))
         return fail(LHYSEA_ERR_ARGUMENT, "bad adjoint arguments");
     if (nsteps > s->hcap) return fail(LHYSEA_ERR_ARGUMENT, "too many steps for the dt history");
-    if (s->p.friction_law != 0) return fail(LHYSEA_ERR_ARGUMENT, "the adjoint implements the Coulomb law");
     for (int g = 0; g < ng; g++)
         if (gauges[g] < 0 || (size_t)gauges[g] >= s->N) return fail(LHYSEA_ERR_ARGUMENT, "gauge cell out of range");
     // a forward pass whose reverse sweep never came is discarded
@@ -828,11 +829,12 @@ int lhysea_adjoint_forward(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const i
     a->store_all = nsnap > nsteps;
     const size_t nslots = a->store_all ? (size_t)nsteps + 1 : (size_t)nsnap;
     a->ngmu = adj_gmu_slots(s->k);
+    a->ngcomp = s->p.friction_law == 1 ? 4 : 1;   // Pouliquen: the four angles' tangents
     if (hipMalloc(&a->snap, nslots * a->state_bytes) != hipSuccess || hipMalloc(&a->P, 6 * N * fsz) != hipSuccess ||
         hipMalloc(&a->ACC, 6 * N * fsz) != hipSuccess || hipMalloc(&a->AB, 6 * N * fsz) != hipSuccess ||
         hipMalloc(&a->LAM, 6 * N * sizeof(double)) != hipSuccess ||
         (!f64 && hipMalloc(&a->LAM2, 6 * N * sizeof(double)) != hipSuccess) ||
-        hipMalloc(&a->gmu, a->ngmu * sizeof(double)) != hipSuccess ||
+        hipMalloc(&a->gmu, a->ngcomp * a->ngmu * sizeof(double)) != hipSuccess ||
         hipMalloc(&a->G, (size_t)nsteps * (ng > 0 ? ng : 1) * sizeof(double)) != hipSuccess)
         return done(fail(LHYSEA_ERR_GPU_MEMORY, "adjoint workspace allocation failed"));
     ADJCHK(hipEventCreate(&a->e0));
@@ -893,8 +895,13 @@ int lhysea_adjoint_forward(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const i
     return LHYSEA_OK;
 }
 
-int lhysea_adjoint_backward(lhysea_sim* s, const double* seeds, double* grad_W0, double* grad_mu,
-                            double* grad_angle1, double* stats)
+}  // extern "C"
+
+// The reverse sweep of the pending adjoint workspace, which it consumes
+// whatever happens: dL/dW0 into grad_W0 (may be null) and the adjoint of the
+// friction coefficient, fr[0] = dL/dmu (Coulomb) or fr[i] = dL/dtan(angle
+// i+1), i = 0..3 (Pouliquen).
+static int adjoint_sweep(lhysea_sim* s, const double* seeds, double* grad_W0, double fr[4], double* stats)
 {
     if (!s) return fail(LHYSEA_ERR_ARGUMENT, "null handle");
     if (!s->adj) return fail(LHYSEA_ERR_ARGUMENT, "no adjoint forward pass to reverse");
@@ -912,7 +919,7 @@ int lhysea_adjoint_backward(lhysea_sim* s, const double* seeds, double* grad_W0,
     if (ng > 0) ADJCHK(hipMemcpy(a->G, seeds, (size_t)nsteps * ng * sizeof(double), hipMemcpyHostToDevice));
     ADJCHK(hipEventRecord(a->e1, s->stream));
     ADJCHK(hipMemsetAsync(a->LAM, 0, 6 * N * sizeof(double), s->stream));
-    ADJCHK(hipMemsetAsync(a->gmu, 0, a->ngmu * sizeof(double), s->stream));
+    ADJCHK(hipMemsetAsync(a->gmu, 0, a->ngcomp * a->ngmu * sizeof(double), s->stream));
     ADJCHK(hook_restore(a, 0));
     ADJCHK(adj_inject(a->LAM, N, a->gauges, ng, a->G + (size_t)(nsteps - 1) * ng, s->stream));
 
@@ -935,15 +942,14 @@ int lhysea_adjoint_backward(lhysea_sim* s, const double* seeds, double* grad_W0,
     ADJCHK(hipGetLastError());
 
     if (grad_W0) ADJCHK(hipMemcpy(grad_W0, a->LAM, 6 * N * sizeof(double), hipMemcpyDeviceToHost));
-    std::vector<double> slots(a->ngmu);
-    ADJCHK(hipMemcpy(slots.data(), a->gmu, a->ngmu * sizeof(double), hipMemcpyDeviceToHost));
-    double gm = 0.0;
-    for (double v : slots) gm += v;
-    if (grad_mu) *grad_mu = gm;
-    if (grad_angle1) {
-        // mu = |tan a| L/H  ->  dmu/da = sign(tan a) (1 + tan^2 a) L/H
-        const double t = std::tan(s->p.angle1);
-        *grad_angle1 = gm * (t >= 0 ? 1.0 : -1.0) * (1.0 + t * t) * (s->p.L / s->p.H);
+    // slot sums in slot order, per component
+    std::vector<double> slots((size_t)a->ngcomp * a->ngmu);
+    ADJCHK(hipMemcpy(slots.data(), a->gmu, slots.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 4; i++) {
+        double v = 0.0;
+        if (i < a->ngcomp)
+            for (size_t k = 0; k < a->ngmu; k++) v += slots[(size_t)i * a->ngmu + k];
+        fr[i] = v;
     }
     if (stats) {
         float r_ms = 0;
@@ -956,6 +962,44 @@ int lhysea_adjoint_backward(lhysea_sim* s, const double* seeds, double* grad_W0,
         stats[5] = (double)revolve_advances(nsteps, a->nsnap);
     }
     return done(LHYSEA_OK);
+}
+
+// dL/d(angle i+1) from the friction-coefficient adjoint of adjoint_sweep()
+static double angle_gradient(const lhysea_params& p, int i, const double fr[4])
+{
+    const double ang[4] = {p.angle1, p.angle2, p.angle3, p.angle4};
+    const double t = std::tan(ang[i]);
+    if (p.friction_law == 1) return fr[i] * (1.0 + t * t);   // d tan(a) / da = 1 + tan^2 a
+    if (i > 0) return 0.0;
+    // mu = |tan a| L/H  ->  dmu/da = sign(tan a) (1 + tan^2 a) L/H
+    return fr[0] * (t >= 0 ? 1.0 : -1.0) * (1.0 + t * t) * (p.L / p.H);
+}
+
+extern "C" {
+
+int lhysea_adjoint_backward(lhysea_sim* s, const double* seeds, double* grad_W0, double* grad_mu,
+                            double* grad_angle1, double* stats)
+{
+    if (s && s->p.friction_law == 1 && grad_mu)
+        return fail(LHYSEA_ERR_ARGUMENT, "dL/dmu belongs to the Coulomb law; under the Pouliquen law "
+                                         "lhysea_adjoint_backward_angles gives the four angles' gradients");
+    double fr[4];
+    const int rc = adjoint_sweep(s, seeds, grad_W0, fr, stats);
+    if (rc) return rc;
+    if (grad_mu) *grad_mu = fr[0];
+    if (grad_angle1) *grad_angle1 = angle_gradient(s->p, 0, fr);
+    return LHYSEA_OK;
+}
+
+int lhysea_adjoint_backward_angles(lhysea_sim* s, const double* seeds, double* grad_W0, double* grad_angles,
+                                   double* stats)
+{
+    double fr[4];
+    const int rc = adjoint_sweep(s, seeds, grad_W0, fr, stats);
+    if (rc) return rc;
+    if (grad_angles)
+        for (int i = 0; i < 4; i++) grad_angles[i] = angle_gradient(s->p, i, fr);
+    return LHYSEA_OK;
 }
 
 int lhysea_adjoint(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const int32_t* gauges, int32_t ng,

@@ -129,19 +129,25 @@ constexpr int kBatchSplit = 0;
 // class table of scenario sid of the batch at its current step (device descriptors in sd)
 hipError_t f32c_batch_classify(const BatchScen& d, const BatchScen* sd_dev, int sid, hipStream_t s);
 
-// Adjoint (lhysea_adjoint.hip)
+// Adjoint (lhysea_adjoint.hip).  gmu: the friction-coefficient adjoint slots,
+// one per workgroup: d/dmu (Coulomb) at gmu[slot], or d/dtan(angle i) of the
+// Pouliquen law (friction_law 1) at gmu[i * gstride + slot], i = 0..3.
 hipError_t adj_reverse_step_f32(const Consts& k, const float* W, const double* H, float* P, float* ACC, float* AB,
-                                double* LAM, double* gmu, const double* dt_hist, int step, hipStream_t s);
+                                double* LAM, double* gmu, size_t gstride, const double* dt_hist, int step,
+                                hipStream_t s);
 // FP32 reverse step as two launches: the recomputation and one tiled kernel
 // for the cell and edge adjoints (lambda^{n+1} in LAMin, lambda^n to LAMout),
 // which also adds the gauge seeds Gk (ng values at the gauge cells; null: none)
 hipError_t adj_reverse_step_f32_fused(const Consts& k, const float* W, const double* H, float* P, float* ACC,
-                                      const double* LAMin, double* LAMout, double* gmu, const double* dt_hist,
-                                      int step, const int* gauges, int ng, const double* Gk, hipStream_t s);
+                                      const double* LAMin, double* LAMout, double* gmu, size_t gstride,
+                                      const double* dt_hist, int step, const int* gauges, int ng, const double* Gk,
+                                      hipStream_t s);
 hipError_t adj_reverse_step_f64(const Consts& k, const double* W, const double* H, double* P, double* ACC,
-                                double* AB, double* LAM, double* gmu, const double* dt_hist, int step, hipStream_t s);
+                                double* AB, double* LAM, double* gmu, size_t gstride, const double* dt_hist, int step,
+                                hipStream_t s);
 hipError_t adj_inject(double* LAM, size_t N, const int* gauges, int ng, const double* G, hipStream_t s);
-// d/dmu slots (one per workgroup of the largest reverse-step launch)
+// friction-coefficient adjoint slots per component (one per workgroup of the
+// largest reverse-step launch)
 size_t adj_gmu_slots(const Consts& k);
 hipError_t adj_set_step(Ctl* ctl, int step, hipStream_t s);
 

@@ -138,6 +138,7 @@ SIGNATURES = {
     "lhysea_tile_classes": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
     "lhysea_adjoint_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, C.c_int32, _dp]),
     "lhysea_adjoint_backward": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
+    "lhysea_adjoint_backward_angles": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
     "lhysea_adjoint": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, C.c_int32, _dp, _dp, _dp, _dp, _dp,
                                  _dp]),
     "lhysea_shallow_water": (C.c_int, [C.POINTER(CParams), _dp, _dp, C.c_double, _dp, _dp, _dp, _ip]),
@@ -335,24 +336,41 @@ class Simulation:
 
     def adjoint_backward(self, seeds):
         """Reverse sweep for the seeds dL/deta (nsteps, ng) of any loss of the
-        gauge series: dL/dW0 (6, ny, nx), dL/dmu, dL/dangle1 (radians)."""
+        gauge series: dL/dW0 (6, ny, nx), dL/dangle of the four friction
+        angles (radians; Coulomb: only angle1 enters), dL/dangle1, and dL/dmu
+        of the Coulomb coefficient (None under the Pouliquen law)."""
         if not getattr(self, "_adj", None):
             raise LhyseaError(3, "no adjoint forward pass to reverse")
         nsteps, ng = self._adj
         seeds = np.ascontiguousarray(seeds, dtype=np.float64).reshape(nsteps, ng)
         gW = np.zeros((6, self.params.ny, self.params.nx))
-        gmu, gang = C.c_double(), C.c_double()
         stats = np.zeros(6)
-        _check(self.lib.lhysea_adjoint_backward(self.h, _dptr(seeds), _dptr(gW), C.byref(gmu), C.byref(gang),
-                                                _dptr(stats)))
+        if self.params.friction_law == 1:
+            gang4 = np.zeros(4)
+            _check(self.lib.lhysea_adjoint_backward_angles(self.h, _dptr(seeds), _dptr(gW), _dptr(gang4),
+                                                           _dptr(stats)))
+            gmu = None
+        else:
+            gm, gang = C.c_double(), C.c_double()
+            _check(self.lib.lhysea_adjoint_backward(self.h, _dptr(seeds), _dptr(gW), C.byref(gm), C.byref(gang),
+                                                    _dptr(stats)))
+            gmu, gang4 = gm.value, np.array([gang.value, 0.0, 0.0, 0.0])
         self._adj = None
-        return {"grad_W0": gW, "grad_mu": gmu.value, "grad_angle1": gang.value,
+        return {"grad_W0": gW, "grad_mu": gmu, "grad_angle1": float(gang4[0]), "grad_angles": gang4,
                 "forward_ms": stats[0], "reverse_ms": stats[1], "advances": int(stats[2]),
                 "reverse_steps": int(stats[3]), "snapshots": int(stats[4]), "optimal_advances": int(stats[5])}
 
     def adjoint(self, nsteps: int, nsnap: int, gauges, obs):
-        """Gauge-misfit loss over nsteps steps from the uploaded state and its
-        gradients: dL/dW0 (6, ny, nx), dL/dmu, dL/dangle1 (radians)."""
+        """Gauge-misfit loss L = sum (eta - obs)^2 over nsteps steps from the
+        uploaded state and its gradients: dL/dW0 (6, ny, nx), dL/dmu (Coulomb),
+        dL/dangle1, and dL/dangle of the four angles (radians)."""
+        if self.params.friction_law == 1:
+            # the forward / backward pair with the misfit's seeds 2 (eta - obs)
+            eta = self.adjoint_forward(nsteps, nsnap, gauges)
+            d = eta - np.asarray(obs, dtype=np.float64).reshape(eta.shape)
+            out = self.adjoint_backward(2.0 * d)
+            out["loss"] = float(np.sum(d * d))
+            return out
         g = np.ascontiguousarray(gauges, dtype=np.int32)
         obs = np.ascontiguousarray(obs, dtype=np.float64).reshape(nsteps, len(g))
         gW = np.zeros((6, self.params.ny, self.params.nx))
@@ -361,7 +379,7 @@ class Simulation:
         _check(self.lib.lhysea_adjoint(self.h, nsteps, nsnap, g.ctypes.data_as(_ip), len(g), _dptr(obs),
                                        C.byref(loss), _dptr(gW), C.byref(gmu), C.byref(gang), _dptr(stats)))
         return {"loss": loss.value, "grad_W0": gW, "grad_mu": gmu.value, "grad_angle1": gang.value,
-                "forward_ms": stats[0], "reverse_ms": stats[1], "advances": int(stats[2]),
+                "grad_angles": np.array([gang.value, 0.0, 0.0, 0.0]), "forward_ms": stats[0], "reverse_ms": stats[1], "advances": int(stats[2]),
                 "reverse_steps": int(stats[3]), "snapshots": int(stats[4]), "optimal_advances": int(stats[5])}
 
 

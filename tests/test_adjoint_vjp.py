@@ -6,8 +6,10 @@ lhysea_adjoint_gen.inc, from csrc/gen/gen_adjoint.py) on 20000 random edges
 and cells each -- wet, thin and dry depths, both layers, speed caps on and
 off, partial sums that drive the positivity limiter -- with random output
 adjoints: every input adjoint within 2e-3 of the central difference wherever
-the one-sided differences agree.  A wrong reverse rule (the square root's
-1/2 planted as 0.45 in a copy of the generated code) fails it.
+the one-sided differences agree; the same for the Pouliquen law's
+adj_edge_pq / adj_cell_pq, the cell's four angle tangents included (states
+at rest, slow and fast: the law's three regimes).  A wrong reverse rule (the
+square root's 1/2 planted as 0.45 in a copy of the generated code) fails it.
 """
 import os
 import shutil
@@ -31,6 +33,8 @@ def test_generated_adjoint_matches_finite_differences(tmp_path):
     assert r.returncode == 0, r.stdout
     for line in r.stdout.splitlines():
         assert int(line.split("checked")[1].split()[0]) > 100000, line
+    tangents = [line for line in r.stdout.splitlines() if "angle tangents" in line]
+    assert tangents and int(tangents[0].split("angle tangents")[1].split(")")[0]) > 1000
 
 
 def test_planted_reverse_rule_is_caught(tmp_path):

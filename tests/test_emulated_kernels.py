@@ -25,8 +25,10 @@ Checks, bit for bit between paths of the library on the same inputs:
 and against the FP64 oracle (C restatement): the emulated FP64 step within
 1e-10 relative L2 over the state, and the discrete adjoint (FP64 four-pass and
 FP32C fused reverse steps) against the oracle's central differences with the
-dt sequence frozen (1e-4 and 2e-2, as on the GPU), with binomial checkpoints
-(restores and class rebuilds in workgroup order) equal to keeping every state.
+dt sequence frozen (1e-4 and 2e-2, as on the GPU), under the Coulomb law and
+under the Pouliquen law (dL/dangle of each of its four angles), with binomial
+checkpoints (restores and class rebuilds in workgroup order) equal to keeping
+every state.
 
 The emulated library is built with -ffp-contract=off, so its arithmetic is not
 that of the gfx950 code object; the GPU tests repeat these checks on the
@@ -57,6 +59,67 @@ def build_emulated(out: str):
     cmd += [os.path.join(CSRC, s) for s in SOURCES]
     cmd += ["-x", "none", os.path.join(HERE, "native", "emu", "emu_rt.cpp"), "-o", out]
     subprocess.run(cmd, check=True)
+
+
+def adjoint_case(L, ob, case, gauge_cells, oracle_params, prec, pouliquen):
+    """Gauge misfit over 12 steps of a flowing 24 x 18 slide against a run from
+    a 10% thicker slide: the adjoint's dL/dW0 at smooth cells and dL/dangle
+    against central differences of the oracle with the dt sequence frozen."""
+    from dataclasses import replace
+    nsteps = 12
+    sc, W, H, prm = case(24, 18, angle=2.0, semi_axes=(5.0, 4.0), precision=prec)
+    if pouliquen:
+        prm = replace(prm, friction_law=1, angle_deg=(2.0, 6.0, 3.0, 8.0))
+    gauges = gauge_cells(sc, 4, seed=3)
+    po = oracle_params(ob, prm)
+    Wt = W.copy()
+    Wt[3] *= 1.1
+    Wt[0] = np.maximum(0.0, (W[0] + W[3]) - Wt[3])
+    obs = ob.run(po, Wt, H, nsteps, gauges=gauges)["gauges"]
+    dts = ob.run(po, W.copy(), H, nsteps, gauges=gauges)["dt_hist"]
+    out = []
+    for nsnap in (4, nsteps + 1):
+        with L.Simulation(prm) as sim:
+            sim.upload(W, H)
+            out.append(sim.adjoint(nsteps, nsnap, gauges, obs))
+    r = out[0]
+
+    def loss(Wx, p=po):
+        g = ob.run_fixed_dt(p, Wx.copy(), H, dts, gauges)
+        return float(((g - obs) ** 2).sum())
+
+    L0 = loss(W)
+    rels, smooth = [], 0
+    rng = np.random.default_rng(0)
+    for c in rng.choice(np.flatnonzero(W[3].ravel() > 5.0), 5, replace=False):
+        j, i = divmod(int(c), prm.nx)
+        for k in (0, 3):
+            eps = 1e-5 * max(1.0, W[k, j, i])
+            Wp, Wm = W.copy(), W.copy()
+            Wp[k, j, i] += eps
+            Wm[k, j, i] -= eps
+            lp, lm = loss(Wp), loss(Wm)
+            fp, fm, fc = (lp - L0) / eps, (L0 - lm) / eps, (lp - lm) / (2 * eps)
+            if abs(fp - fm) <= 1e-3 * abs(fc):
+                smooth += 1
+                rels.append(abs(r["grad_W0"][k, j, i] - fc) / abs(fc))
+    # dL/dangle: Coulomb perturbs the four angles together (only angle1 enters),
+    # Pouliquen each angle alone
+    names = ("angle1", "angle2", "angle3", "angle4")
+    fa = []
+    for which in ((names,) if not pouliquen else tuple((a,) for a in names)):
+        pp, pm = oracle_params(ob, prm), oracle_params(ob, prm)
+        for q, sgn in ((pp, 1), (pm, -1)):
+            for a in which:
+                setattr(q, a, getattr(q, a) + sgn * 1e-6)
+        fa.append((loss(W, pp) - loss(W, pm)) / 2e-6)
+    ga = [r["grad_angle1"]] if not pouliquen else [float(x) for x in r["grad_angles"]]
+    big = max(abs(x) for x in fa)
+    angle_rel = max(abs(g - f) / abs(f) for g, f in zip(ga, fa) if abs(f) >= 1e-3 * big)
+    return {"loss_rel": abs(r["loss"] - L0) / L0, "smooth": smooth, "max_rel": max(rels) if rels else None,
+            "angle_rel": angle_rel, "angles_fd": fa, "angles_adj": ga, "advances": r["advances"],
+            "checkpoints_equal_all_states": bool(np.array_equal(out[0]["grad_W0"], out[1]["grad_W0"])
+                                                 and np.array_equal(out[0]["grad_angles"], out[1]["grad_angles"]))}
 
 
 def run_cases(lib: str) -> dict:
@@ -161,53 +224,12 @@ def run_cases(lib: str) -> dict:
 
     # discrete adjoint (FP32C: the fused reverse kernel; FP64: the four-pass
     # reverse step) against central differences of the oracle with the dt
-    # sequence frozen, and binomial checkpointing against keeping every state
+    # sequence frozen, and binomial checkpointing against keeping every state;
+    # under the Coulomb law and under the Pouliquen law (the gradients of its
+    # four angles one by one)
     for prec in (L.FP64, L.FP32C):
-        nsteps = 12
-        sc, W, H, prm = case(24, 18, angle=2.0, semi_axes=(5.0, 4.0), precision=prec)
-        gauges = gauge_cells(sc, 4, seed=3)
-        po = oracle_params(ob, prm)
-        Wt = W.copy()
-        Wt[3] *= 1.1
-        Wt[0] = np.maximum(0.0, (W[0] + W[3]) - Wt[3])
-        obs = ob.run(po, Wt, H, nsteps, gauges=gauges)["gauges"]
-        dts = ob.run(po, W.copy(), H, nsteps, gauges=gauges)["dt_hist"]
-        out = []
-        for nsnap in (4, nsteps + 1):
-            with L.Simulation(prm) as sim:
-                sim.upload(W, H)
-                out.append(sim.adjoint(nsteps, nsnap, gauges, obs))
-        r = out[0]
-
-        def loss(Wx, p=po):
-            g = ob.run_fixed_dt(p, Wx.copy(), H, dts, gauges)
-            return float(((g - obs) ** 2).sum())
-
-        L0 = loss(W)
-        rels, smooth = [], 0
-        rng = np.random.default_rng(0)
-        for c in rng.choice(np.flatnonzero(W[3].ravel() > 5.0), 5, replace=False):
-            j, i = divmod(int(c), prm.nx)
-            for k in (0, 3):
-                eps = 1e-5 * max(1.0, W[k, j, i])
-                Wp, Wm = W.copy(), W.copy()
-                Wp[k, j, i] += eps
-                Wm[k, j, i] -= eps
-                lp, lm = loss(Wp), loss(Wm)
-                fp, fm, fc = (lp - L0) / eps, (L0 - lm) / eps, (lp - lm) / (2 * eps)
-                if abs(fp - fm) <= 1e-3 * abs(fc):
-                    smooth += 1
-                    rels.append(abs(r["grad_W0"][k, j, i] - fc) / abs(fc))
-        pp, pm = oracle_params(ob, prm), oracle_params(ob, prm)
-        for q, sgn in ((pp, 1), (pm, -1)):
-            for a in ("angle1", "angle2", "angle3", "angle4"):
-                setattr(q, a, getattr(q, a) + sgn * 1e-6)
-        fa = (loss(W, pp) - loss(W, pm)) / 2e-6
-        res[f"adjoint_{prec}"] = {
-            "loss_rel": abs(r["loss"] - L0) / L0, "smooth": smooth, "max_rel": max(rels) if rels else None,
-            "angle_rel": abs(r["grad_angle1"] - fa) / abs(fa), "advances": r["advances"],
-            "checkpoints_equal_all_states": bool(np.array_equal(out[0]["grad_W0"], out[1]["grad_W0"])
-                                                 and out[0]["grad_angle1"] == out[1]["grad_angle1"])}
+        res[f"adjoint_{prec}"] = adjoint_case(L, ob, case, gauge_cells, oracle_params, prec, False)
+        res[f"adjoint_pq_{prec}"] = adjoint_case(L, ob, case, gauge_cells, oracle_params, prec, True)
     return res
 
 
@@ -269,13 +291,17 @@ def test_run_until_split_in_workgroup_order(emulated):
         assert r["same"] and r["steps"][0] > 5 and r["steps"][1] == 7, r
 
 
+@pytest.mark.parametrize("law", ["", "pq_"])
 @pytest.mark.parametrize("prec,tol", [(0, 1e-4), (1, 2e-2)])
-def test_adjoint_in_workgroup_order_matches_oracle_fd(emulated, prec, tol):
-    r = emulated[f"adjoint_{prec}"]
+def test_adjoint_in_workgroup_order_matches_oracle_fd(emulated, prec, tol, law):
+    r = emulated[f"adjoint_{law}{prec}"]
     assert r["loss_rel"] <= (1e-9 if prec == 0 else 5e-2)
     assert r["smooth"] >= 6 and r["max_rel"] <= tol, r
     assert r["angle_rel"] <= tol, r
     assert r["advances"] > 0 and r["checkpoints_equal_all_states"], r
+    if law:
+        # every angle of the law moves the loss on this flowing slide
+        assert len(r["angles_fd"]) == 4 and min(abs(x) for x in r["angles_fd"]) > 0, r
 
 
 if __name__ == "__main__":

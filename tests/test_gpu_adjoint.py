@@ -15,7 +15,9 @@ kinks and jumps: where the one-sided differences agree the adjoint is
 checked against the central difference; at a branch point it must equal the
 one-sided derivative of one side (the side whose branches the forward took).
 """
+import ctypes as C
 import math
+from dataclasses import replace
 
 import numpy as np
 import pytest
@@ -33,8 +35,14 @@ def say(*a):
     print(*a, flush=True)
 
 
-def setup(oracle, precision, nx=48, ny=36, N=40):
+# the Pouliquen law's four angles (stop / start pairs) on the flowing slide
+PQ_ANGLES = (2.0, 6.0, 3.0, 8.0)
+
+
+def setup(oracle, precision, nx=48, ny=36, N=40, pouliquen=False):
     sc, W, H, prm = case(nx, ny, angle=2.0, semi_axes=(10.0, 7.0), precision=precision)
+    if pouliquen:
+        prm = replace(prm, friction_law=1, angle_deg=PQ_ANGLES)
     gauges = gauge_cells(sc, 6, seed=3)
     po = oracle_params(oracle, prm)
     Wp = W.copy()
@@ -301,3 +309,107 @@ def test_series_autograd_any_loss_vs_oracle_fd(oracle):
           - loss_np(oracle.run_fixed_dt(pm, W.copy(), H, dts, gauges))) / (2 * da)
     say(f"angle: autograd {float(ang.grad):.6e} fd {fd:.6e}")
     assert abs(float(ang.grad) - fd) <= 1e-4 * abs(fd)
+
+
+@pytest.mark.parametrize("precision,tol", [(L.FP64, 1e-4), (L.FP32C, 2e-2)])
+def test_pouliquen_adjoint_matches_oracle_fd(oracle, precision, tol):
+    """The Pouliquen law (src/GPU/Arista_kernel.cu:20-58) in the adjoint:
+    dL/dW0 at smooth slide cells and dL/dangle of each of the four angles
+    against the oracle's central differences, at the bars of the Coulomb
+    test; 8 snapshots equal every state kept."""
+    N = 40
+    sc, W, H, prm, po, gauges, obs, dts = setup(oracle, precision, N=N, pouliquen=True)
+    say(f"Pouliquen adjoint precision={precision} 48x36 N={N}, angles {PQ_ANGLES}")
+    out = []
+    for nsnap in (8, N + 1):
+        with L.Simulation(prm) as sim:
+            sim.upload(W, H)
+            out.append(sim.adjoint(N, nsnap, gauges, obs))
+    res = out[0]
+    assert res["grad_mu"] is None and res["reverse_steps"] == N
+    assert np.array_equal(out[0]["grad_W0"], out[1]["grad_W0"])
+    assert np.array_equal(out[0]["grad_angles"], out[1]["grad_angles"])
+    L0 = fd_loss(oracle, po, W, H, dts, gauges, obs)
+    assert abs(res["loss"] - L0) <= (1e-9 if precision == L.FP64 else 5e-2) * L0
+    rng = np.random.default_rng(5)
+    smooth = 0
+    for c in rng.choice(np.flatnonzero(W[3].ravel() > 5.0), 8, replace=False):
+        j, i = divmod(int(c), prm.nx)
+        for k in (0, 3):
+            eps = 1e-5 * max(1.0, W[k, j, i])
+            Wp, Wm = W.copy(), W.copy()
+            Wp[k, j, i] += eps
+            Wm[k, j, i] -= eps
+            lp = fd_loss(oracle, po, Wp, H, dts, gauges, obs)
+            lm = fd_loss(oracle, po, Wm, H, dts, gauges, obs)
+            fp, fm, fc = (lp - L0) / eps, (L0 - lm) / eps, (lp - lm) / (2 * eps)
+            ad = res["grad_W0"][k, j, i]
+            if abs(fp - fm) <= 1e-3 * abs(fc) + 1e-12:
+                smooth += 1
+                assert abs(ad - fc) <= tol * abs(fc) + 1e-12, (c, k, ad, fc)
+            else:
+                assert min(abs(ad - fp), abs(ad - fm)) <= max(tol, 1e-3) * max(abs(fp), abs(fm)) + 1e-12
+    assert smooth >= 8
+    # each angle alone; a step whose one-sided differences disagree straddles a
+    # branch (a Coulomb stop / go flip somewhere in the 40 steps: at 1e-6 rad
+    # for angle1 on this slide), so a smaller step is tried, and where both
+    # straddle one the adjoint must equal the derivative of one side
+    fds = []
+    for a in ("angle1", "angle2", "angle3", "angle4"):
+        for da in (1e-6, 1e-7):
+            pp, pm = oracle_params(oracle, prm), oracle_params(oracle, prm)
+            setattr(pp, a, getattr(pp, a) + da)
+            setattr(pm, a, getattr(pm, a) - da)
+            lp = fd_loss(oracle, pp, W, H, dts, gauges, obs)
+            lm = fd_loss(oracle, pm, W, H, dts, gauges, obs)
+            fp, fm, fc = (lp - L0) / da, (L0 - lm) / da, (lp - lm) / (2 * da)
+            if abs(fp - fm) <= 1e-3 * abs(fc) + 1e-12:
+                break
+        fds.append((fc, fp, fm))
+    say("dL/dangle adjoint", list(res["grad_angles"]), "fd (central, forward, backward)", fds)
+    big = max(abs(f[0]) for f in fds)
+    checked = 0
+    for g, (fc, fp, fm) in zip(res["grad_angles"], fds):
+        if max(abs(fc), abs(fp), abs(fm)) < 1e-5 * big:
+            # at the differences' noise (~1e-7 here): the adjoint is as small
+            assert abs(g) <= 1e-4 * big, (g, fc)
+        elif abs(fp - fm) > 1e-3 * abs(fc) + 1e-12:
+            assert min(abs(g - fp), abs(g - fm)) <= tol * max(abs(fp), abs(fm)), (g, fp, fm)
+        else:
+            checked += 1
+            assert abs(g - fc) <= tol * abs(fc), (g, fc)
+    assert checked >= 2
+    assert res["grad_angle1"] == res["grad_angles"][0]
+
+
+def test_pouliquen_adjoint_pair_and_torch_wrapper(oracle):
+    """Under the Pouliquen law: lhysea_adjoint_backward refuses dL/dmu (the
+    Coulomb coefficient's) and keeps the workspace, which
+    lhysea_adjoint_backward_angles then reverses; the torch wrapper with a
+    (4,) angle tensor returns the same four gradients."""
+    N = 30
+    sc, W, H, prm, po, gauges, obs, dts = setup(oracle, L.FP64, N=N, pouliquen=True)
+    obs = obs[:N]
+    with L.Simulation(prm) as sim:
+        sim.upload(W, H)
+        ref = sim.adjoint(N, 6, gauges, obs)
+    with L.Simulation(prm) as sim:
+        sim.upload(W, H)
+        eta = sim.adjoint_forward(N, 6, gauges)
+        seeds = np.ascontiguousarray(2.0 * (eta - obs))
+        gmu = C.c_double()
+        rc = sim.lib.lhysea_adjoint_backward(sim.h, seeds.ctypes.data_as(C.POINTER(C.c_double)), None,
+                                             C.byref(gmu), None, None)
+        assert rc == 3, rc     # LHYSEA_ERR_ARGUMENT
+        assert "Pouliquen" in sim.lib.lhysea_last_error().decode()
+        r = sim.adjoint_backward(seeds)     # the workspace is still there
+    assert np.array_equal(r["grad_W0"], ref["grad_W0"]) and np.array_equal(r["grad_angles"], ref["grad_angles"])
+    spec = MisfitSpec(prm, W, H, gauges, obs, N, 6)
+    h2 = torch.tensor(W[3], dtype=torch.float64, requires_grad=True)
+    ang = torch.tensor([math.radians(a) for a in PQ_ANGLES], dtype=torch.float64, requires_grad=True)
+    loss = GaugeMisfit.apply(h2, ang, spec)
+    loss.backward()
+    say("torch dL/dangles", ang.grad.tolist())
+    assert ang.grad.shape == (4,)
+    assert np.array_equal(ang.grad.numpy(), ref["grad_angles"])
+    assert np.array_equal(h2.grad.numpy(), ref["grad_W0"][3])
