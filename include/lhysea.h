@@ -129,6 +129,13 @@ int lhysea_get_gauges(lhysea_sim *s, double *out, int32_t *nrec);
 int lhysea_dt_history(lhysea_sim *s, double *out, int32_t max_steps, int32_t *nrec);
 /* LHYSEA_KERNEL_FUSED or LHYSEA_KERNEL_FOURPASS */
 int lhysea_set_kernel(lhysea_sim *s, int32_t mode);
+/* Form of the FP32C adjoint's reverse step (no counterpart in the reference):
+ * LHYSEA_KERNEL_FUSED (default; one tiled kernel after the recomputation) or
+ * LHYSEA_KERNEL_FOURPASS (cell adjoint, then one launch per edge class with
+ * the partial-sum adjoints in HBM).  Gradients agree to rounding; the
+ * four-pass form exists as a cross-check.  The FP64 reverse step is four-pass
+ * either way. */
+int lhysea_set_adjoint_kernel(lhysea_sim *s, int32_t mode);
 /* FP32C step by tile class (no counterpart in the reference; same results
  * either way).  The grid is cut into 62 x 14 class tiles; a tile whose cells
  * and 1-cell halo hold no sediment (h2 = 0, q2 = 0) takes a reduced kernel

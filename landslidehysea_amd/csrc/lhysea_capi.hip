@@ -79,6 +79,7 @@ struct lhysea_sim {
     hipStream_t stream = nullptr;
     int cur = 0;                 // buffer index holding the current state
     int mode = MODE_FUSED;
+    int adj_mode = MODE_FUSED;   // FP32C reverse step: fused tile kernel or four passes
     Buf64 b64{};
     Buf32 b32{};
     Ctl* ctl = nullptr;
@@ -602,6 +603,14 @@ int lhysea_set_kernel(lhysea_sim* s, int32_t mode)
     return LHYSEA_OK;
 }
 
+int lhysea_set_adjoint_kernel(lhysea_sim* s, int32_t mode)
+{
+    if (!s || (mode != MODE_FUSED && mode != MODE_FOURPASS)) return fail(LHYSEA_ERR_ARGUMENT, "bad kernel mode");
+    if (s->adj) return adj_busy();
+    s->adj_mode = mode;
+    return LHYSEA_OK;
+}
+
 int lhysea_set_tile_split(lhysea_sim* s, int32_t mode)
 {
     if (!s || mode < -1 || mode > 1) return fail(LHYSEA_ERR_ARGUMENT, "bad handle or split mode");
@@ -771,6 +780,10 @@ hipError_t hook_reverse(void* c, int k)
     if (s->p.precision == LHYSEA_FP64) {
         e = adj_reverse_step_f64(s->k, (const double*)W, s->b64.H, (double*)a->P, (double*)a->ACC, (double*)a->AB,
                                  a->LAM, a->gmu, a->ngmu, s->dt_hist, k, s->stream);
+        if (e == hipSuccess && Gk) e = adj_inject(a->LAM, s->N, a->gauges, a->ng, Gk, s->stream);
+    } else if (s->adj_mode == MODE_FOURPASS) {
+        e = adj_reverse_step_f32(s->k, (const float*)W, s->b32.H, (float*)a->P, (float*)a->ACC, (float*)a->AB, a->LAM,
+                                 a->gmu, a->ngmu, s->dt_hist, k, s->stream);
         if (e == hipSuccess && Gk) e = adj_inject(a->LAM, s->N, a->gauges, a->ng, Gk, s->stream);
     } else {
         // lambda^{n+1} in LAM, lambda^n (with the seeds) into LAM2, then swap

@@ -133,6 +133,7 @@ SIGNATURES = {
     "lhysea_get_gauges": (C.c_int, [C.c_void_p, _dp, _ip]),
     "lhysea_dt_history": (C.c_int, [C.c_void_p, _dp, C.c_int32, _ip]),
     "lhysea_set_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lhysea_set_adjoint_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
     "lhysea_time_steps": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
     "lhysea_set_tile_split": (C.c_int, [C.c_void_p, C.c_int32]),
     "lhysea_tile_classes": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
@@ -323,6 +324,11 @@ class Simulation:
         _check(self.lib.lhysea_time_steps(self.h, nsteps, C.byref(tot), C.byref(ker) if per_kernel else None))
         return tot.value, (ker.value if per_kernel else None)
 
+
+    def set_adjoint_kernel(self, mode: int):
+        """Form of the FP32C adjoint's reverse step: KERNEL_FUSED (default) or
+        KERNEL_FOURPASS (a cross-check; gradients agree to rounding)."""
+        _check(self.lib.lhysea_set_adjoint_kernel(self.h, mode))
 
     def adjoint_forward(self, nsteps: int, nsnap: int, gauges) -> np.ndarray:
         """Forward pass of the adjoint pair from the uploaded state: returns the

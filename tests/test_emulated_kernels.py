@@ -28,7 +28,8 @@ FP32C fused reverse steps) against the oracle's central differences with the
 dt sequence frozen (1e-4 and 2e-2, as on the GPU), under the Coulomb law and
 under the Pouliquen law (dL/dangle of each of its four angles), with binomial
 checkpoints (restores and class rebuilds in workgroup order) equal to keeping
-every state.
+every state, and the FP32C fused reverse kernel equal to the four-pass reverse
+step to rounding (1e-6 of the largest gradient).
 
 The emulated library is built with -ffp-contract=off, so its arithmetic is not
 that of the gfx950 code object; the GPU tests repeat these checks on the
@@ -83,6 +84,16 @@ def adjoint_case(L, ob, case, gauge_cells, oracle_params, prec, pouliquen):
             sim.upload(W, H)
             out.append(sim.adjoint(nsteps, nsnap, gauges, obs))
     r = out[0]
+    fourpass_rel = None
+    if prec == L.FP32C:
+        # the four-pass form of the reverse step against the fused kernel
+        with L.Simulation(prm) as sim:
+            sim.set_adjoint_kernel(L.KERNEL_FOURPASS)
+            sim.upload(W, H)
+            r4 = sim.adjoint(nsteps, 4, gauges, obs)
+        gmax = np.abs(r["grad_W0"]).max()
+        fourpass_rel = max(float(np.abs(r4["grad_W0"] - r["grad_W0"]).max() / gmax),
+                           float(np.abs(r4["grad_angles"] - r["grad_angles"]).max() / np.abs(r["grad_angles"]).max()))
 
     def loss(Wx, p=po):
         g = ob.run_fixed_dt(p, Wx.copy(), H, dts, gauges)
@@ -118,6 +129,7 @@ def adjoint_case(L, ob, case, gauge_cells, oracle_params, prec, pouliquen):
     angle_rel = max(abs(g - f) / abs(f) for g, f in zip(ga, fa) if abs(f) >= 1e-3 * big)
     return {"loss_rel": abs(r["loss"] - L0) / L0, "smooth": smooth, "max_rel": max(rels) if rels else None,
             "angle_rel": angle_rel, "angles_fd": fa, "angles_adj": ga, "advances": r["advances"],
+            "fourpass_rel": fourpass_rel,
             "checkpoints_equal_all_states": bool(np.array_equal(out[0]["grad_W0"], out[1]["grad_W0"])
                                                  and np.array_equal(out[0]["grad_angles"], out[1]["grad_angles"]))}
 
@@ -302,6 +314,9 @@ def test_adjoint_in_workgroup_order_matches_oracle_fd(emulated, prec, tol, law):
     if law:
         # every angle of the law moves the loss on this flowing slide
         assert len(r["angles_fd"]) == 4 and min(abs(x) for x in r["angles_fd"]) > 0, r
+    if prec == 1:
+        # FP32C: the fused reverse kernel equals the four-pass reverse step to rounding
+        assert r["fourpass_rel"] <= 1e-6, r
 
 
 if __name__ == "__main__":

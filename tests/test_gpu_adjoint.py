@@ -413,3 +413,29 @@ def test_pouliquen_adjoint_pair_and_torch_wrapper(oracle):
     assert ang.grad.shape == (4,)
     assert np.array_equal(ang.grad.numpy(), ref["grad_angles"])
     assert np.array_equal(h2.grad.numpy(), ref["grad_W0"][3])
+
+
+@pytest.mark.parametrize("pouliquen", [False, True])
+def test_fp32c_fused_reverse_equals_fourpass(oracle, pouliquen):
+    """The FP32C reverse step's fused tile kernel (one 38 x 20 tile and its
+    2-cell cone per workgroup, everything in LDS) against its four-pass form
+    (cell adjoint, then one launch per edge class with the partial-sum
+    adjoints in HBM): the same generated edge and cell adjoints in the same
+    class order with the same float and double accumulations, so dL/dW0 is
+    equal bit for bit; dL/dangle differs only by the grouping of the
+    workgroups' slot sums."""
+    N = 40
+    sc, W, H, prm, po, gauges, obs, dts = setup(oracle, L.FP32C, N=N, pouliquen=pouliquen)
+    res = {}
+    for mode in (L.KERNEL_FUSED, L.KERNEL_FOURPASS):
+        with L.Simulation(prm) as sim:
+            sim.set_adjoint_kernel(mode)
+            sim.upload(W, H)
+            res[mode] = sim.adjoint(N, 8, gauges, obs)
+    a, b = res[L.KERNEL_FUSED], res[L.KERNEL_FOURPASS]
+    assert a["loss"] == b["loss"]
+    gmax = np.abs(a["grad_W0"]).max()
+    dw = float(np.abs(a["grad_W0"] - b["grad_W0"]).max() / gmax)
+    da = float(np.abs(a["grad_angles"] - b["grad_angles"]).max() / np.abs(a["grad_angles"]).max())
+    say(f"fused vs four-pass reverse ({'Pouliquen' if pouliquen else 'Coulomb'}): dW0 {dw:.3e}, dangles {da:.3e}")
+    assert gmax > 0 and np.array_equal(a["grad_W0"], b["grad_W0"]) and da <= 1e-12
