@@ -4,8 +4,8 @@
 // One reverse step n (state W^n in the working buffer, frozen dt^n, adjoint
 // lambda^{n+1} of the step's output state):
 //   1. recompute the four edge classes (even y, odd y, even x, odd x) with the
-//      generated forward edge, keeping the partial h-sums each class reads
-//      (the positivity limiter's inputs) and the final flux sums;
+//      forward step's own tile kernel, keeping the partial h-sums each class
+//      reads (the positivity limiter's inputs) and the final flux sums;
 //   2. cell adjoint: lambda^{n+1} -> adjoint of the flux sums, the direct part
 //      of lambda^n, and d/dmu (Coulomb coefficient) of the cell update;
 //   3. edge adjoints in reverse class order: each edge adds its state
@@ -163,48 +163,6 @@ __device__ __forceinline__ EC<F> edge_consts(const Consts& k, double dt)
     return c;
 }
 
-// 1. forward recomputation of one class, keeping the partial sums it reads
-// PQ: the Pouliquen law (friction_law 1) in this and the other adjoint kernels
-template <typename F, bool PQ>
-__global__ void __launch_bounds__(256) k_recompute(Consts k, SV<F> sv, const double* H, Work<F> w,
-                                                   const double* dt_hist, int step, int cls)
-{
-    const size_t N = sv.N;
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    EdgeRef r;
-    if (!edge_ref(k, e, cls, r)) return;
-    const EC<F> c = edge_consts<F>(k, dt_hist[step]);
-    const F a0h1 = w.ACC[r.c0], a0h2 = w.ACC[3 * N + r.c0];
-    const F a1h1 = r.interior ? w.ACC[r.c1] : F(0), a1h2 = r.interior ? w.ACC[3 * N + r.c1] : F(0);
-    if (cls > 0) {
-        F* P = w.P + (size_t)(cls - 1) * 2 * N;
-        P[r.c0] = a0h1;
-        P[N + r.c0] = a0h2;
-        if (r.interior) {
-            P[r.c1] = a1h1;
-            P[N + r.c1] = a1h2;
-        }
-    }
-    EdgeIn<F> in;
-    edge_inputs(sv, H, r, in);
-    F fm[6], fp[6];
-    if constexpr (PQ)
-        adj_edge_pq_fwd<F>((F)c.eps, (F)c.eps4, (F)c.g, (F)c.gp, (F)c.r, (F)c.mu, (F)c.dt, (F)c.area, (F)r.len,
-                           (F)r.n1x, (F)r.n1y, r.interior ? F(1) : F(0), in.H0, in.H1, c.tan1, c.tan2, c.tan3, c.tan4,
-                           c.LoverH, in.h10, in.h20, in.h11, in.h21, in.q0[0], in.q0[1], in.q0[2], in.q0[3], in.q1[0],
-                           in.q1[1], in.q1[2], in.q1[3], a0h1, a0h2, a1h1, a1h2, fm[0], fm[1], fm[2], fm[3], fm[4],
-                           fm[5], fp[0], fp[1], fp[2], fp[3], fp[4], fp[5]);
-    else
-        adj_edge_fwd<F>((F)c.eps, (F)c.eps4, (F)c.g, (F)c.gp, (F)c.r, (F)c.mu, (F)c.dt, (F)c.area, (F)r.len,
-                        (F)r.n1x, (F)r.n1y, r.interior ? F(1) : F(0), in.H0, in.H1, in.h10, in.h20, in.h11, in.h21,
-                        in.q0[0], in.q0[1], in.q0[2], in.q0[3], in.q1[0], in.q1[1], in.q1[2], in.q1[3], a0h1, a0h2,
-                        a1h1, a1h2, fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fp[0], fp[1], fp[2], fp[3], fp[4],
-                        fp[5]);
-    for (int f = 0; f < 6; f++) w.ACC[f * N + r.c0] -= fm[f];
-    if (r.interior)
-        for (int f = 0; f < 6; f++) w.ACC[f * N + r.c1] -= fp[f];
-}
-
 __device__ __forceinline__ double block_sum256(double v)
 {
     __shared__ double red[4];
@@ -218,7 +176,7 @@ __device__ __forceinline__ double block_sum256(double v)
 }
 
 // 2. cell adjoint: lambda^{n+1} -> flux-sum adjoints, direct lambda^n, d/dmu
-//    (Pouliquen: d/dtan of the four angles)
+//    (PQ, the Pouliquen law: d/dtan of the four angles)
 template <typename F, bool PQ>
 __global__ void __launch_bounds__(256) k_cell_adj(Consts k, SV<F> sv, Work<F> w, const double* dt_hist, int step)
 {
@@ -807,17 +765,13 @@ hipError_t reverse_step_t(const Consts& k, const F* W, const double* H, F* P, F*
     long ne[4];
     for (int cls = 0; cls < 4; cls++)
         ne[cls] = (cls < 2) ? (long)((k.ny - cls) / 2 + 1) * k.nx : (long)((k.nx - (cls - 2)) / 2 + 1) * k.ny;
-    if constexpr (sizeof(F) == 4) {
-        // the forward step's own tile kernel recomputes the partial and final sums
-        hipError_t e = f32c_recompute(k, (const float*)W, H, dt_hist, step, (float*)P, (float*)ACC, s);
-        if (e != hipSuccess) return e;
-    } else {
-        hipError_t e = hipMemsetAsync(ACC, 0, 6 * N * sizeof(F), s);
-        if (e != hipSuccess) return e;
-        for (int cls = 0; cls < 4; cls++)
-            hipLaunchKernelGGL((adj::k_recompute<F, PQ>), dim3((unsigned)((ne[cls] + 255) / 256)), dim3(256), 0, s, k,
-                               sv, H, w, dt_hist, step, cls);
-    }
+    // the forward step's own tile kernel recomputes the partial and final sums
+    hipError_t e;
+    if constexpr (sizeof(F) == 4)
+        e = f32c_recompute(k, W, H, dt_hist, step, P, ACC, s);
+    else
+        e = f64_recompute(k, W, H, dt_hist, step, P, ACC, s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL((adj::k_cell_adj<F, PQ>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, k, sv, w, dt_hist,
                        step);
     for (int cls = 3; cls >= 0; cls--)

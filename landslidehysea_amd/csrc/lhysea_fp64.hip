@@ -280,11 +280,15 @@ __device__ __forceinline__ void accumulate(double (&sA)[7][TY][CW], int r, int c
 // masks of this thread's cells that keep sediment, the class tile at
 // (cx0, cy0).  hc (optional): control block whose health word records the
 // cell updates' health bits (HEALTH_*) as those of step hs.
-template <int TX, int TY, int NT>
+// RECOMP: instead of updating the state, write the partial h1/h2 sums that
+// classes 2..4 read (P, [3][2][N]) and the final flux sums (ACC, [6][N]) of
+// the tile's cells: the forward recomputation of the discrete adjoint.
+template <int TX, int TY, int NT, bool RECOMP = false>
 __device__ __forceinline__ double tile_step64(const Consts& k, const double* __restrict__ W, double* __restrict__ Wn,
                                               const double* __restrict__ H, double dt, double tnow,
                                               double* __restrict__ emax, int i0, int j0, GaugeOut go, int* smask,
-                                              int cx0, int cy0, Ctl* hc = nullptr, int hs = 0)
+                                              int cx0, int cy0, Ctl* hc = nullptr, int hs = 0,
+                                              double* __restrict__ P = nullptr, double* __restrict__ ACC = nullptr)
 {
     constexpr int CW = TX + 2;              // staged columns
     constexpr int RW = TY + 2;              // staged rows
@@ -312,6 +316,20 @@ __device__ __forceinline__ double tile_step64(const Consts& k, const double* __r
         for (int f = 0; f < 7; f++) sA[f][r][c] = 0.0;
     }
     __syncthreads();
+
+    // RECOMP: the tile's own h1/h2 partial sums after class cls (read by class cls + 1)
+    auto store_partials = [&](int cls) {
+        double* Pc = P + (size_t)cls * 2 * N;
+        for (int e = tid; e < TX * TY; e += NT) {
+            const int r = e / TX, c = e % TX;
+            const int gj = j0 + r, gi = i0 + c;
+            if (gj >= ny || gi >= nx) continue;
+            const size_t g = (size_t)gj * nx + gi;
+            Pc[g] = sA[0][r][c + 1];
+            Pc[N + g] = sA[3][r][c + 1];
+        }
+        __syncthreads();   // before the next class adds to the sums
+    };
 
     // y-edge classes C1 (even) then C2 (odd), on staged columns -1..TX
 #pragma unroll 1
@@ -363,6 +381,7 @@ __device__ __forceinline__ double tile_step64(const Consts& k, const double* __r
             }
         }
         __syncthreads();
+        if (RECOMP) store_partials(par);
     }
 
     // x-edge classes C3 (even) then C4 (odd), on tile rows 0..TY-1
@@ -413,6 +432,19 @@ __device__ __forceinline__ double tile_step64(const Consts& k, const double* __r
             }
         }
         __syncthreads();
+        if (RECOMP && par == 0) store_partials(2);
+    }
+
+    if (RECOMP) {
+        for (int e = tid; e < TX * TY; e += NT) {
+            const int r = e / TX, c = e % TX;
+            const int gj = j0 + r, gi = i0 + c;
+            if (gj >= ny || gi >= nx) continue;
+            const size_t g = (size_t)gj * nx + gi;
+#pragma unroll
+            for (int f = 0; f < 6; f++) ACC[f * N + g] = sA[f][r][c + 1];
+        }
+        return 0.0;
     }
 
     // cell update of the tile's own cells
@@ -666,6 +698,27 @@ __global__ void __launch_bounds__(NT) k_fused_all(Consts k, const double* __rest
 }
 
 // Sediment-free tiles of a step, one block per tile (general ones leave).
+// Forward recomputation of the discrete adjoint's reverse step: every class
+// tile through tile_step64<RECOMP> (the step's own edge arithmetic), writing
+// the partial sums and final flux sums instead of the new state.
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_recompute(Consts k, const double* __restrict__ W,
+                                                  const double* __restrict__ H, const double* __restrict__ dt_seq,
+                                                  int step, double* __restrict__ P, double* __restrict__ ACC,
+                                                  int tiles_x, int tiles_y)
+{
+    const int nb = tiles_x * tiles_y;
+    int b;
+    {
+        const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+        const int q = nb >> 3, rem = nb & 7;
+        b = xcd * q + (xcd < rem ? xcd : rem) + slot;
+    }
+    const int tx = b % tiles_x, ty = b / tiles_x;
+    tile_step64<TX, TY, NT, true>(k, W, nullptr, H, dt_seq[step], 0.0, nullptr, tx * TX, ty * TY,
+                                  GaugeOut{nullptr, 0, nullptr}, nullptr, 0, 0, nullptr, 0, P, ACC);
+}
+
 template <int TX, int TY, int NT>
 __global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const double* __restrict__ W, double* __restrict__ Wn,
                                                    const double* __restrict__ H, StepArgs a, TileCls tc)
@@ -837,6 +890,16 @@ hipError_t f64_classify(const Consts& k, const double* W, const StepArgs& a, hip
 
 // Split (a.split): the sediment-free launch, then the general tiles and the
 // step's close; otherwise every tile general, then the close.
+hipError_t f64_recompute(const Consts& k, const double* W, const double* H, const double* dt_seq, int step, double* P,
+                         double* ACC, hipStream_t s)
+{
+    const int tiles_x = (k.nx + f64::FTX - 1) / f64::FTX;
+    const int tiles_y = (k.ny + f64::FTY - 1) / f64::FTY;
+    hipLaunchKernelGGL((f64::k_recompute<f64::FTX, f64::FTY, f64::FNT>), dim3((unsigned)(tiles_x * tiles_y)),
+                       dim3(f64::FNT), 0, s, k, W, H, dt_seq, step, P, ACC, tiles_x, tiles_y);
+    return hipGetLastError();
+}
+
 hipError_t f64_step_fused(const Consts& k, const double* W, double* Wn, const double* H, const StepArgs& a,
                           hipStream_t s)
 {

@@ -92,6 +92,9 @@ int f32c_split_min_bare();
 hipError_t f32c_classify(const Consts& k, const float* W, const StepArgs& a, hipStream_t s);
 // partial h sums read by classes 2..4 (P, [3][2][N]) and flux sums (ACC, [6][N])
 // of the step from W with dt = dt_seq[step], by the fused tile kernel
+// FP64 forward recomputation for the adjoint (partial sums P [3][2][N], flux sums ACC [6][N])
+hipError_t f64_recompute(const Consts& k, const double* W, const double* H, const double* dt_seq, int step, double* P,
+                         double* ACC, hipStream_t s);
 hipError_t f32c_recompute(const Consts& k, const float* W, const double* H, const double* dt_seq, int step, float* P,
                           float* ACC, hipStream_t s);
 
