@@ -178,10 +178,13 @@ static void choose_split(lhysea_sim* s, const Ctl& c)
 // Tile classes of the current FP32C state at the current step; needed after
 // every change of the state or step count other than a step.
 // With choose, the launch split is chosen from the new classes (a host sync).
-static hipError_t classify(lhysea_sim* s, bool choose)
+// Tile classes of the current state (or of the state at src, in the current
+// state's layout).
+static hipError_t classify(lhysea_sim* s, bool choose, const void* src = nullptr)
 {
-    hipError_t e = (s->p.precision == LHYSEA_FP32C) ? f32c_classify(s->k, s->b32.W[s->cur], step_args(s), s->stream)
-                                                     : f64_classify(s->k, s->b64.W[s->cur], step_args(s), s->stream);
+    hipError_t e = (s->p.precision == LHYSEA_FP32C)
+                       ? f32c_classify(s->k, src ? (const float*)src : s->b32.W[s->cur], step_args(s), s->stream)
+                       : f64_classify(s->k, src ? (const double*)src : s->b64.W[s->cur], step_args(s), s->stream);
     if (!choose) return e;
     Ctl c;
     if (e == hipSuccess) e = hipMemcpyAsync(&c, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->stream);
@@ -696,6 +699,7 @@ struct lhysea_adj_work {
     size_t ngmu = 0;
     int ngcomp = 1;
     const char* W = nullptr; // stored state of the reverse step (all states kept), else null
+    const char* src = nullptr; // restored snapshot that the next advance or reverse step reads in place
     int* gauges = nullptr;
     double* gser = nullptr;  // gauge series of the forward pass
     double* G = nullptr;     // [nsteps][ng] seeds dL/deta
@@ -745,26 +749,41 @@ void* state_ptr(lhysea_sim* s) { return s->p.precision == LHYSEA_FP64 ? (void*)s
 hipError_t hook_advance(void* c, int k0, int n)
 {
     lhysea_adj_work* a = (lhysea_adj_work*)c;
-    hipError_t e = adj_set_step(a->s->ctl, k0, a->s->stream);
-    a->s->step_host = k0;
+    lhysea_sim* s = a->s;
+    hipError_t e = adj_set_step(s->ctl, k0, s->stream);
+    s->step_host = k0;
     // the restored state's tile classes (the launch split of the forward pass is kept)
-    if (e == hipSuccess) e = classify(a->s, false);
+    if (e == hipSuccess) e = classify(s, false, a->src);
     if (e != hipSuccess) return e;
-    return launch_steps(a->s, n) == LHYSEA_OK ? hipSuccess : hipErrorUnknown;
+    if (a->src && n > 0) {
+        // the first step reads the restored snapshot in place; the snapshot
+        // stays intact (the step writes the other state buffer)
+        const int nxt = 1 - s->cur;
+        e = launch_one(s, a->src, state_buf(s, nxt), launch_args(s, nullptr, nullptr));
+        if (e != hipSuccess) return e;
+        s->cur = nxt;
+        a->src = nullptr;
+        n--;
+    }
+    return launch_steps(s, n) == LHYSEA_OK ? hipSuccess : hipErrorUnknown;
 }
 
 hipError_t hook_store(void* c, int slot)
 {
     lhysea_adj_work* a = (lhysea_adj_work*)c;
-    return hipMemcpyAsync(a->snap + (size_t)slot * a->state_bytes, state_ptr(a->s), a->state_bytes,
-                          hipMemcpyDeviceToDevice, a->s->stream);
+    char* dst = a->snap + (size_t)slot * a->state_bytes;
+    const void* from = a->src ? (const void*)a->src : state_ptr(a->s);
+    if (from == dst) return hipSuccess;
+    return hipMemcpyAsync(dst, from, a->state_bytes, hipMemcpyDeviceToDevice, a->s->stream);
 }
 
+// A restore copies nothing: the next advance or reverse step reads the
+// snapshot slot in place.
 hipError_t hook_restore(void* c, int slot)
 {
     lhysea_adj_work* a = (lhysea_adj_work*)c;
-    return hipMemcpyAsync(state_ptr(a->s), a->snap + (size_t)slot * a->state_bytes, a->state_bytes,
-                          hipMemcpyDeviceToDevice, a->s->stream);
+    a->src = a->snap + (size_t)slot * a->state_bytes;
+    return hipSuccess;
 }
 
 hipError_t hook_reverse(void* c, int k)
@@ -772,8 +791,9 @@ hipError_t hook_reverse(void* c, int k)
     lhysea_adj_work* a = (lhysea_adj_work*)c;
     lhysea_sim* s = a->s;
     hipError_t e;
-    // the state W^k: the working buffer, or its stored slot
-    const void* W = a->W ? (const void*)a->W : state_ptr(s);
+    // the state W^k: its stored slot (every state kept, or a restored
+    // snapshot), else the working buffer
+    const void* W = a->W ? (const void*)a->W : a->src ? (const void*)a->src : state_ptr(s);
     // the seeds of the gauge series at step k (rows of G: after step k) join
     // lambda^k as it is formed
     const double* Gk = (k >= 1 && a->ng > 0) ? a->G + (size_t)(k - 1) * a->ng : nullptr;
