@@ -143,6 +143,30 @@ Consts lhysea::make_consts(const lhysea_params& p)
     k.vmax2 = p.vmax2;
     k.peso = p.peso;
     k.beta = p.beta;
+    ConstsF32& f = k.f;
+    f.eps = (float)k.eps_h;
+    f.eps4 = f.eps * f.eps * f.eps * f.eps;
+    f.g = (float)k.g;
+    f.gp = (float)(k.g * (1.0 - k.r));
+    f.r = (float)k.r;
+    f.mu = (float)k.mu;
+    f.peso = (float)k.peso;
+    f.area = (float)k.area;
+    f.len_x = (float)k.dy;
+    f.len_y = (float)k.dx;
+    f.igp = (float)(1.0 / (k.g * (1.0 - k.r)));
+    f.ten_eps = (float)(10.0 / k.eps_h);
+    f.ieps = (float)(1.0 / k.eps_h);
+    f.mfc = (float)k.mfc;
+    f.mf0 = (float)k.mf0;
+    f.mfs = (float)k.mfs;
+    f.vmax1 = (float)k.vmax1;
+    f.vmax2 = (float)k.vmax2;
+    f.tan1 = (float)k.tan1;
+    f.tan2 = (float)k.tan2;
+    f.tan3 = (float)k.tan3;
+    f.tan4 = (float)k.tan4;
+    f.LoverH = (float)k.LoverH;
     return k;
 }
 

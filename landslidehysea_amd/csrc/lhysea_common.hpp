@@ -12,6 +12,18 @@
 
 namespace lhysea {
 
+// FP32 forms of the per-run constants read by the FP32-compensated kernels:
+// each is (float) of the FP64 constant or of the FP64 expression named, set
+// once on the host so that no kernel wave spends instructions deriving them.
+struct ConstsF32 {
+    float eps, eps4;          // eps_h, eps^4 (FP32 products)
+    float g, gp, r, mu, peso; // gp = g (1 - r)
+    float area, len_x, len_y; // dx dy; edge lengths dy (x-edges), dx (y-edges)
+    float igp, ten_eps, ieps; // 1 / (g (1 - r)), 10 / eps_h, 1 / eps_h
+    float mfc, mf0, mfs, vmax1, vmax2;
+    float tan1, tan2, tan3, tan4, LoverH;
+};
+
 // Per-run constants, derived on the host from the reference's parameter set
 // (src/GPU/main.cxx:157-164, src/GPU/Problema.cxx:226-296).
 struct Consts {
@@ -25,6 +37,7 @@ struct Consts {
     double tan1, tan2, tan3, tan4, LoverH;   // Pouliquen law
     double mfc, mf0, mfs, vmax1, vmax2;
     double peso, beta;
+    ConstsF32 f;
 };
 
 // Device-side step control block (one per simulation).

@@ -10,6 +10,7 @@
 // reference's order with LDS accumulators, the compensated cell update, and
 // a wavefront-shuffle max of the wave-speed sums for the lagged dt.
 #include <algorithm>
+#include <type_traits>
 #include "lhysea_internal.hpp"
 #include "lhysea_fp32.hpp"
 #include "lhysea_tiles.hpp"
@@ -43,49 +44,67 @@ __device__ __forceinline__ double step_dt(const Ctl* ctl, const double* dt_seq)
     return dt_seq ? dt_seq[ctl->step] : ctl->dt_cur;
 }
 
-__device__ __forceinline__ K32 make_k32(const Consts& k, double dt)
+// FP32 constants of a step with time step dt: (float)dt, 1 / (dt + EPS) of
+// the positivity limiter and dt / area.  FP64 divisions: a kernel derives
+// them once per workgroup where it can (tile_step_bare).
+struct DtF {
+    float dtf, idt_eps, bq;
+};
+
+__device__ __forceinline__ DtF dt_floats(const Consts& k, double dt)
 {
+    DtF d;
+    d.dtf = (float)dt;
+    d.idt_eps = (float)(1.0 / ((double)d.dtf + (double)LH_EPSF));
+    d.bq = (float)(dt / k.area);
+    return d;
+}
+
+__device__ __forceinline__ K32 make_k32(const Consts& k, const DtF& d)
+{
+    const ConstsF32& f = k.f;
     K32 c;
-    c.eps = (float)k.eps_h;
-    c.eps4 = c.eps * c.eps * c.eps * c.eps;
-    c.g = (float)k.g;
-    c.gp = (float)(k.g * (1.0 - k.r));
-    c.r = (float)k.r;
-    c.mu = (float)k.mu;
-    c.peso = (float)k.peso;
-    c.dtf = (float)dt;
-    c.area = (float)k.area;
-    c.len_x = (float)k.dy;
-    c.len_y = (float)k.dx;
-    c.igp = (float)(1.0 / (k.g * (1.0 - k.r)));
-    c.ten_eps = (float)(10.0 / k.eps_h);
-    c.idt_eps = (float)(1.0 / ((double)c.dtf + (double)LH_EPSF));
-    c.bq = (float)(dt / k.area);
+    c.eps = f.eps;
+    c.eps4 = f.eps4;
+    c.g = f.g;
+    c.gp = f.gp;
+    c.r = f.r;
+    c.mu = f.mu;
+    c.peso = f.peso;
+    c.dtf = d.dtf;
+    c.area = f.area;
+    c.len_x = f.len_x;
+    c.len_y = f.len_y;
+    c.igp = f.igp;
+    c.ten_eps = f.ten_eps;
+    c.idt_eps = d.idt_eps;
+    c.bq = d.bq;
     c.pouliquen = k.friction_law == 1;
-    c.tan1 = (float)k.tan1;
-    c.tan2 = (float)k.tan2;
-    c.tan3 = (float)k.tan3;
-    c.tan4 = (float)k.tan4;
-    c.LoverH = (float)k.LoverH;
+    c.tan1 = f.tan1;
+    c.tan2 = f.tan2;
+    c.tan3 = f.tan3;
+    c.tan4 = f.tan4;
+    c.LoverH = f.LoverH;
     return c;
 }
 
-__device__ __forceinline__ KCell make_kcell(const Consts& k, const K32& kc, double dt)
+__device__ __forceinline__ KCell make_kcell(const Consts& k, const K32& kc)
 {
+    const ConstsF32& f = k.f;
     KCell kk;
     kk.eps = kc.eps;
     kk.eps4 = kc.eps4;
     kk.r = kc.r;
     kk.g = kc.g;
     kk.mu = kc.mu;
-    kk.mfc = (float)k.mfc;
-    kk.mf0 = (float)k.mf0;
-    kk.mfs = (float)k.mfs;
-    kk.vmax1 = (float)k.vmax1;
-    kk.vmax2 = (float)k.vmax2;
+    kk.mfc = f.mfc;
+    kk.mf0 = f.mf0;
+    kk.mfs = f.mfs;
+    kk.vmax1 = f.vmax1;
+    kk.vmax2 = f.vmax2;
     kk.dtf = kc.dtf;
-    kk.valf = (float)(dt / k.area);
-    kk.ieps = (float)(1.0 / k.eps_h);
+    kk.valf = kc.bq;
+    kk.ieps = f.ieps;
     kk.ten_eps = kc.ten_eps;
     kk.pouliquen = kc.pouliquen;
     kk.tan1 = kc.tan1;
@@ -102,7 +121,7 @@ __global__ void __launch_bounds__(256) k_dt0(Consts k, const float* __restrict__
     const int nx = k.nx, ny = k.ny;
     const size_t N = (size_t)nx * ny;
     const long cidx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const K32 c = make_k32(k, 0.0);
+    const K32 c = make_k32(k, dt_floats(k, 0.0));
     double A = 0.0;
     if (cidx < (long)N) {
         const int i = (int)(cidx % nx), j = (int)(cidx / nx);
@@ -168,6 +187,10 @@ __device__ __forceinline__ void accumulate(float (&sA)[7][TY][CW], int r, int c,
     sA[6][r][c] += lam;
 }
 
+// word 3 of a raw buffer resource on gfx9 targets (gfx950 among them):
+// 32-bit data format, no swizzle, no index stride
+constexpr int BUF_DWORD3 = 0x00020000;
+
 constexpr int FTX = 62, FTY = 14, FNT = 512;
 // general sub-tiles: width, threads, and the fixed grid walking them (four
 // workgroups per CU of an MI355X)
@@ -214,7 +237,7 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
 
     const int nx = k.nx, ny = k.ny;
     const size_t N = (size_t)nx * ny;
-    const K32 kc = make_k32(k, dt);
+    const K32 kc = make_k32(k, dt_floats(k, dt));
     const int tid = threadIdx.x;
 
     // HBM loads are issued early and all at once: the q1 compensations the
@@ -437,7 +460,7 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
     }
 
     // compensated cell update of the tile's own cells
-    const KCell kk = make_kcell(k, kc, dt);
+    const KCell kk = make_kcell(k, kc);
     double Amax = 0.0;
 #pragma unroll
     for (int it = 0; it < CU_IT; it++) {
@@ -518,23 +541,63 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
 // only the three layer-1 planes and their compensations are written.  hc
 // (optional): control block whose health word records the cell updates'
 // health bits as those of step s.
-template <int TX, int TY, int NT>
+// BUF: the state and depth are read and written with buffer instructions
+// (plane offset in a scalar register, a 32-bit byte offset per cell), so that
+// no vector instruction forms an address; valid while the 12 FP32 planes stay
+// below 4 GiB (buf_fits).  Otherwise 64-bit addresses.
+template <int TX, int TY, int NT, bool BUF = false>
 __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __restrict__ W, float* __restrict__ Wn,
                                                const double* __restrict__ H, double dt, double tnow,
                                                double* __restrict__ emax, int i0, int j0, const int* cls, int s,
                                                int skip_from, GaugeOut go, double& Amax, Ctl* hc = nullptr)
 {
     static_assert(TX % 2 == 0 && TY % 2 == 0, "tile origins must stay even");
+    using Idx = typename std::conditional<BUF, unsigned, size_t>::type;
     constexpr int CW = TX + 2, RW = TY + 2, CWP = CW + 1;
     __shared__ double sH[RW][CWP];
     __shared__ double sh1[RW][CWP];
     __shared__ float sq[2][RW][CWP];
     __shared__ float sA[4][TY][CWP];   // dh1, dq1x, dq1y, wave-speed sum
 
+    __shared__ float sdt[2];           // the step's 1 / (dt + EPS) and dt / area
+
     const int nx = k.nx, ny = k.ny;
     const size_t N = (size_t)nx * ny;
-    const K32 kc = make_k32(k, dt);
     const int tid = threadIdx.x;
+    // FP32 plane p at cell g of the old / new state, depth at cell g
+    [[maybe_unused]] __amdgpu_buffer_rsrc_t rW, rWn, rH;
+    [[maybe_unused]] unsigned pb = 0u;
+    if constexpr (BUF) {
+        pb = (unsigned)N * 4u;
+        rW = __builtin_amdgcn_make_buffer_rsrc((void*)W, (short)0, (int)(12u * pb), BUF_DWORD3);
+        rWn = __builtin_amdgcn_make_buffer_rsrc((void*)Wn, (short)0, (int)(12u * pb), BUF_DWORD3);
+        rH = __builtin_amdgcn_make_buffer_rsrc((void*)H, (short)0, (int)(2u * pb), BUF_DWORD3);
+    }
+    auto ldW = [&](int p, Idx g) -> float {
+        if constexpr (BUF)
+            return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rW, g * 4u, p * pb, 0));
+        else
+            return W[p * N + g];
+    };
+    auto ldH = [&](Idx g) -> double {
+        if constexpr (BUF)
+            return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rH, g * 8u, 0, 0));
+        else
+            return H[g];
+    };
+    auto stW = [&](int p, Idx g, float v) {
+        if constexpr (BUF)
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rWn, g * 4u, p * pb, 0);
+        else
+            Wn[p * N + g] = v;
+    };
+    if (tid < 64) {                    // one wave derives them for the workgroup
+        const DtF d = dt_floats(k, dt);
+        if (tid == 0) {
+            sdt[0] = d.idt_eps;
+            sdt[1] = d.bq;
+        }
+    }
 
     constexpr int CU_IT = (TX * TY + NT - 1) / NT, ST_IT = (RW * CW + NT - 1) / NT;
     float lo1x[CU_IT], lo1y[CU_IT];
@@ -545,9 +608,9 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
         const int r = e / TX, c = e % TX;
         lo1x[it] = lo1y[it] = 0.0f;
         if (e < TX * TY && j0 + r < ny && i0 + c < nx) {
-            const size_t g = (size_t)(j0 + r) * nx + (i0 + c);
-            lo1x[it] = W[7 * N + g];
-            lo1y[it] = W[8 * N + g];
+            const Idx g = (Idx)(j0 + r) * nx + (i0 + c);
+            lo1x[it] = ldW(7, g);
+            lo1y[it] = ldW(8, g);
         }
     }
     {
@@ -561,11 +624,11 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
             const int gj = j0 + r - 1, gi = i0 + c - 1;
             in[it] = e < RW * CW && gj >= 0 && gj < ny && gi >= 0 && gi < nx;
             if (in[it]) {
-                const size_t g = (size_t)gj * nx + gi;
-                vH[it] = H[g];
-                vh1[it] = (double)W[g] + (double)W[6 * N + g];
-                vq[it][0] = W[N + g];
-                vq[it][1] = W[2 * N + g];
+                const Idx g = (Idx)gj * nx + gi;
+                vH[it] = ldH(g);
+                vh1[it] = (double)ldW(0, g) + (double)ldW(6, g);
+                vq[it][0] = ldW(1, g);
+                vq[it][1] = ldW(2, g);
             }
         }
         const int v = *cls;
@@ -588,6 +651,7 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
         for (int f = 0; f < 4; f++) sA[f][r][c] = 0.0f;
     }
     __syncthreads();
+    const K32 kc = make_k32(k, DtF{(float)dt, sdt[0], sdt[1]});
 
     auto loadW = [&](int r, int c, float* w) {
         w[0] = (float)sh1[r][c];
@@ -694,7 +758,7 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
     }
 
     // cell update of the tile's own cells (layer 2 stays zero)
-    const KCell kk = make_kcell(k, kc, dt);
+    const KCell kk = make_kcell(k, kc);
     Amax = 0.0;
     unsigned hb = 0u;
 #pragma unroll
@@ -704,7 +768,7 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
         const int r = e / TX, c = e % TX;
         const int gj = j0 + r, gi = i0 + c;
         if (gj >= ny || gi >= nx) continue;
-        const size_t g = (size_t)gj * nx + gi;
+        const Idx g = (Idx)gj * nx + gi;
         Pair o[3];
         const double h1d = sh1[r + 1][c + 1];
         o[0].hi = (float)h1d; o[0].lo = (float)(h1d - (double)o[0].hi);
@@ -722,14 +786,14 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
         cell_update1(kk, o, ac, &hb);
 #pragma unroll
         for (int f = 0; f < 3; f++) {
-            Wn[f * N + g] = o[f].hi;
-            Wn[(6 + f) * N + g] = o[f].lo;
+            stW(f, g, o[f].hi);
+            stW(6 + f, g, o[f].lo);
         }
         if (zero2) {
 #pragma unroll
             for (int f = 0; f < 3; f++) {
-                Wn[(3 + f) * N + g] = 0.0f;
-                Wn[(9 + f) * N + g] = 0.0f;
+                stW(3 + f, g, 0.0f);
+                stW(9 + f, g, 0.0f);
             }
         }
         // the new h1 replaces this cell's staged one (read by no other thread now)
@@ -748,7 +812,7 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
 // contiguous (blocks are dispatched round-robin over the 8 XCDs, so XCD x
 // runs blocks x, x+8, ...; giving it one run of tiles keeps shared halo rows
 // in its L2).
-template <int TX, int TY, int NT>
+template <int TX, int TY, int NT, bool BUF>
 __global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const float* __restrict__ W, float* __restrict__ Wn,
                                                    const double* __restrict__ H, StepArgs a, TileCls tc)
 {
@@ -765,7 +829,7 @@ __global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const float* __rest
     const double dt = step_dt(a.ctl, a.dt_seq);
     const GaugeOut go{a.gauges, a.ng, (a.gseries && s < a.gcap) ? a.gseries + (size_t)s * a.ng : nullptr};
     double Amax;
-    if (!tile_step_bare<TX, TY, NT>(k, W, Wn, H, dt, a.ctl->t, a.emax, tx * TX, ty * TY, tc.cls + (s & 1) * nb + b,
+    if (!tile_step_bare<TX, TY, NT, BUF>(k, W, Wn, H, dt, a.ctl->t, a.emax, tx * TX, ty * TY, tc.cls + (s & 1) * nb + b,
                                     s, a.wn_prev ? a.ctl->cls_from : 0x7fffffff, go, Amax, a.ctl))
         return;
     block_max_atomic<NT>(Amax, &a.ctl->maxA_next);
@@ -967,7 +1031,7 @@ __global__ void __launch_bounds__(NT) k_batch(const BatchScen* __restrict__ sd, 
 // the general class tiles of every scenario cut into sub-tiles, each
 // scenario walked by its own group of gps workgroups.  Classes as in the
 // single-simulation step, per scenario.
-template <int TX, int TY, int NT>
+template <int TX, int TY, int NT, bool BUF>
 __global__ void __launch_bounds__(NT) k_batch_bare(const BatchScen* __restrict__ sd, const unsigned* __restrict__ tab)
 {
     const unsigned e = tab[blockIdx.x];
@@ -981,7 +1045,7 @@ __global__ void __launch_bounds__(NT) k_batch_bare(const BatchScen* __restrict__
     const int cur = step & 1;
     const Consts k = d.k;
     double Amax;
-    if (!tile_step_bare<TX, TY, NT>(k, gmem(d.W[cur]), gmem(d.W[cur ^ 1]), gmem(d.H), ctl->dt_cur, ctl->t,
+    if (!tile_step_bare<TX, TY, NT, BUF>(k, gmem(d.W[cur]), gmem(d.W[cur ^ 1]), gmem(d.H), ctl->dt_cur, ctl->t,
                                     d.emax ? gmem(d.emax) : nullptr, tx * TX, ty * TY,
                                     gmem(d.cls) + (step & 1) * nct + tile, step, ctl->cls_from,
                                     GaugeOut{nullptr, 0, nullptr}, Amax, ctl))
@@ -1201,8 +1265,12 @@ hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const dou
     hipError_t e = a.ev0 ? hipEventRecord(a.ev0, s) : hipSuccess;
     if (e != hipSuccess) return e;
     if (a.split) {
-        hipLaunchKernelGGL((f32::k_fused_bare<f32::FTX, f32::FTY, f32::FNT>), dim3((unsigned)nt), dim3(f32::FNT), 0,
-                           s, k, W, Wn, H, a, tc);
+        if (f32c_buf_fits((size_t)k.nx * k.ny))
+            hipLaunchKernelGGL((f32::k_fused_bare<f32::FTX, f32::FTY, f32::FNT, true>), dim3((unsigned)nt),
+                               dim3(f32::FNT), 0, s, k, W, Wn, H, a, tc);
+        else
+            hipLaunchKernelGGL((f32::k_fused_bare<f32::FTX, f32::FTY, f32::FNT, false>), dim3((unsigned)nt),
+                               dim3(f32::FNT), 0, s, k, W, Wn, H, a, tc);
         hipLaunchKernelGGL((f32::k_fused_gen<f32::GSTX, f32::FTY, f32::GNT>), dim3(f32::GGRID), dim3(f32::GNT), 0, s,
                            k, W, Wn, H, a, tc);
         if (a.ev1 && (e = hipEventRecord(a.ev1, s)) != hipSuccess) return e;
@@ -1214,6 +1282,11 @@ hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const dou
     }
     return hipGetLastError();
 }
+
+// The sediment-free kernels address the state with buffer instructions when
+// its 12 FP32 planes of N cells stay below 4 GiB (N < 89,478,486, about
+// 9459 x 9459); larger grids take 64-bit addresses.
+bool f32c_buf_fits(size_t N) { return N < ((size_t)1 << 32) / 48; }
 
 // measured on an MI355X (single slide scenarios, step kernels with and without
 // the split): FP32C 1024^2 (1238 sediment-free tiles) 47.2 vs 47.3 us, 1536^2
@@ -1243,14 +1316,19 @@ hipError_t f32c_batch_classify(const BatchScen& d, const BatchScen* sd_dev, int 
     return hipGetLastError();
 }
 
-hipError_t f32c_batch_step(int tile, const BatchScen* sd, int nscen, const unsigned* tab, int nblocks, hipStream_t s)
+hipError_t f32c_batch_step(int tile, const BatchScen* sd, int nscen, const unsigned* tab, int nblocks, bool buf,
+                           hipStream_t s)
 {
     switch (tile) {
     case kBatchSplit: {
         // tab lists every 62 x 14 class tile; a fixed grid walks the general
         // sub-tiles of all scenarios
-        hipLaunchKernelGGL((f32::k_batch_bare<f32::FTX, f32::FTY, f32::FNT>), dim3(nblocks), dim3(f32::FNT), 0, s, sd,
-                           tab);
+        if (buf)
+            hipLaunchKernelGGL((f32::k_batch_bare<f32::FTX, f32::FTY, f32::FNT, true>), dim3(nblocks), dim3(f32::FNT),
+                               0, s, sd, tab);
+        else
+            hipLaunchKernelGGL((f32::k_batch_bare<f32::FTX, f32::FTY, f32::FNT, false>), dim3(nblocks),
+                               dim3(f32::FNT), 0, s, sd, tab);
         hipLaunchKernelGGL((f32::k_batch_gen<f32::GSTX, f32::FTY, f32::GNT>), dim3(f32::GGRID), dim3(f32::GNT), 0, s,
                            sd, nscen);
         break;

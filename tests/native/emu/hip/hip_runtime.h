@@ -17,6 +17,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <cstdio>
 #include <functional>
 #include <mutex>
 #include <vector>
@@ -101,6 +102,22 @@ inline float __expf(float x) { return std::exp(x); }
 inline float __powf(float a, float b) { return std::pow(a, b); }
 inline float __builtin_amdgcn_logf(float x) { return std::log2(x); }
 inline float __builtin_amdgcn_exp2f(float x) { return std::exp2(x); }
+// raw buffer resources: base pointer and size in bytes; an access outside
+// [0, size) aborts (the kernels never rely on the hardware's range check)
+struct __amdgpu_buffer_rsrc_t { char* base; unsigned size; };
+inline __amdgpu_buffer_rsrc_t __builtin_amdgcn_make_buffer_rsrc(void* p, short, int n, int) {
+    return __amdgpu_buffer_rsrc_t{(char*)p, (unsigned)n}; }
+inline char* emu_buf_at(const __amdgpu_buffer_rsrc_t& r, unsigned voff, unsigned soff, unsigned bytes) {
+    const unsigned long long o = (unsigned long long)voff + soff;
+    if (o + bytes > r.size) { std::fprintf(stderr, "buffer access out of range: %llu + %u > %u\n", o, bytes, r.size); std::abort(); }
+    return r.base + o;
+}
+inline unsigned __builtin_amdgcn_raw_buffer_load_b32(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, int) {
+    unsigned v; std::memcpy(&v, emu_buf_at(r, voff, soff, 4), 4); return v; }
+inline unsigned long long __builtin_amdgcn_raw_buffer_load_b64(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, int) {
+    unsigned long long v; std::memcpy(&v, emu_buf_at(r, voff, soff, 8), 8); return v; }
+inline void __builtin_amdgcn_raw_buffer_store_b32(unsigned v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, int) {
+    std::memcpy(emu_buf_at(r, voff, soff, 4), &v, 4); }
 extern "C" float fmaf(float, float, float);
 inline float emu_fmaf(float a, float b, float c) { return ::fmaf(a, b, c); }
 #define __builtin_fmaf(a, b, c) emu_fmaf(a, b, c)
