@@ -132,17 +132,43 @@ __device__ __forceinline__ float friction32(const KC& k, float h1, float u1, flo
     return muf * k.LoverH;
 }
 
+// Edge normals are axis-aligned: AX 0 is an x-edge with normal (sn, 0), AX 1
+// a y-edge with normal (0, sn), sn = +-1.  With the normal's zero component
+// dropped the projections and rotations below equal the general forms
+// (q . n, q . t with t = (-ny, nx), and back) exactly, up to the sign of
+// zero results; the tangential flux F[h] u.t does not depend on sn (the two
+// sign flips cancel), so it is formed without it.
+// Normal component of the discharge (qx, qy) = (W[f], W[f + 1]).
+template <int AX>
+__device__ __forceinline__ float qn(const float* W, int f, float sn)
+{
+    return sn * W[f + AX];
+}
+
+// The edge's fluxes in the x/y frame from the normal fluxes Fm, Fp, the
+// tangential flux Ft = F[h] u.t (all per unit length) and lsn = len * sn:
+// n-frame (F, Ft) -> (F nx - Ft ny, F ny + Ft nx), and F+ rotated with -Ft.
+template <int AX>
+__device__ __forceinline__ void rotate_out(float* Mn, float* P, float Fm, float Fp, float Ft, float len, float lsn)
+{
+    const float lt = len * Ft;
+    Mn[1 + AX] = lsn * Fm;
+    Mn[2 - AX] = lt;
+    P[1 + AX] = lsn * Fp;
+    P[2 - AX] = -lt;
+}
+
 // One edge (src/GPU/Arista_kernel.cu:324-645) with FP32 arithmetic and FP64
 // hydrostatic terms.  W0/W1: FP32 states (ghost when !interior); hd*: full
-// precision h1/h2 of both cells; H0/H1 depths.  Coulomb law only.
+// precision h1/h2 of both cells; H0/H1 depths; normal of axis AX and sign sn.
+template <int AX>
 __device__ __forceinline__ bool edge(const K32& k, const float* W0, const float* W1, double h10, double h20,
-                                     double h11, double h21, double H0, double H1, float n1x, float n1y,
-                                     float len, float a0h1, float a0h2, float a1h1, float a1h2, bool interior,
-                                     EdgeFlux& out)
+                                     double h11, double h21, double H0, double H1, float sn, float len, float a0h1,
+                                     float a0h2, float a1h1, float a1h2, bool interior, EdgeFlux& out)
 {
     const float eps = k.eps, eps4 = k.eps4, g = k.g, r = k.r;
-    float w00 = W0[0], w01 = W0[1] * n1x + W0[2] * n1y, w02 = W0[3], w03 = W0[4] * n1x + W0[5] * n1y;
-    float w10 = W1[0], w11 = W1[1] * n1x + W1[2] * n1y, w12 = W1[3], w13 = W1[4] * n1x + W1[5] * n1y;
+    float w00 = W0[0], w01 = qn<AX>(W0, 1, sn), w02 = W0[3], w03 = qn<AX>(W0, 4, sn);
+    float w10 = W1[0], w11 = qn<AX>(W1, 1, sn), w12 = W1[3], w13 = qn<AX>(W1, 4, sn);
 
     // dry/wet treatment with FP64 depth comparisons (src/GPU/Arista_kernel.cu:301-321)
     if ((w02 < eps) && (H1 - h21 > H0)) w13 = 0.0f;
@@ -242,23 +268,20 @@ __device__ __forceinline__ bool edge(const K32& k, const float* W0, const float*
     float Fm0 = M0 + (w01 - V0), Fm1 = M1 + (FL1 - V1), Fm2 = M2 + (w03 - V2), Fm3 = M3 + (FL3 - V3);
     float Fp1 = M1 + (V1 - FR1), Fp3 = M3 + (V3 - FR3);
 
-    // tangential upwinding (src/GPU/Arista_kernel.cu:506-532)
+    // tangential upwinding (src/GPU/Arista_kernel.cu:506-532): the upwind
+    // cell's tangential velocity, up to the normal's sign (see rotate_out)
     float ut1 = 0.0f, ut2 = 0.0f;
-    if (!(fabsf(Fm0) < LH_EPSF)) {
-        ut1 = (Fm0 > 0.0f) ? (W0[2] * n1x - W0[1] * n1y) * s00 : (W1[2] * n1x - W1[1] * n1y) * s10;
-    }
-    if (!(fabsf(Fm2) < LH_EPSF)) {
-        ut2 = (Fm2 > 0.0f) ? (W0[5] * n1x - W0[4] * n1y) * s02 : (W1[5] * n1x - W1[4] * n1y) * s12;
-    }
-    float T1 = Fm0 * ut1, T2 = Fm2 * ut2;
+    if (!(fabsf(Fm0) < LH_EPSF)) ut1 = (Fm0 > 0.0f) ? W0[2 - AX] * s00 : W1[2 - AX] * s10;
+    if (!(fabsf(Fm2) < LH_EPSF)) ut2 = (Fm2 > 0.0f) ? W0[5 - AX] * s02 : W1[5 - AX] * s12;
     float* P = out.Fp;
     float* Mn = out.Fm;
-    Mn[0] = len * Fm0; Mn[1] = len * (Fm1 * n1x - T1 * n1y); Mn[2] = len * (Fm1 * n1y + T1 * n1x);
-    Mn[3] = len * Fm2; Mn[4] = len * (Fm3 * n1x - T2 * n1y); Mn[5] = len * (Fm3 * n1y + T2 * n1x);
+    const float lsn = len * sn;
+    Mn[0] = len * Fm0;
+    Mn[3] = len * Fm2;
     P[0] = -Mn[0];                                  // F+[h] = -F-[h]
-    P[1] = len * (Fp1 * n1x + T1 * n1y); P[2] = len * (Fp1 * n1y - T1 * n1x);
     P[3] = -Mn[3];
-    P[4] = len * (Fp3 * n1x + T2 * n1y); P[5] = len * (Fp3 * n1y - T2 * n1x);
+    rotate_out<AX>(Mn, P, Fm1, Fp1, Fm0 * ut1, len, lsn);
+    rotate_out<AX>(Mn + 3, P + 3, Fm3, Fp3, Fm2 * ut2, len, lsn);
 
     // positivity with the partial accumulators (src/GPU/Arista_kernel.cu:554-609)
     // P[h] = -Mn[h]: at most one of the two outflow candidates exists per layer
@@ -309,13 +332,14 @@ __device__ __forceinline__ void net_zero_pair(float& Dx, float& Dw, float* aut2)
 //    internal eigenvalue pair (0, 0) vanish; the PVM takes the two-wave or
 //    Rusanov form because h2ij < eps;
 //  * eig(FLAG 1) of a cell uses s(f0 + f2) = s(w) of that cell's water.
+template <int AX>
 __device__ __forceinline__ bool edge1(const K32& k, const float* W0, const float* W1, double h10, double h11,
-                                      double H0, double H1, float n1x, float n1y, float len, float a0h1,
-                                      float a1h1, bool interior, EdgeFlux& out)
+                                      double H0, double H1, float sn, float len, float a0h1, float a1h1,
+                                      bool interior, EdgeFlux& out)
 {
     const float eps = k.eps, eps4 = k.eps4, g = k.g;
-    float w00 = W0[0], w01 = W0[1] * n1x + W0[2] * n1y;
-    float w10 = W1[0], w11 = W1[1] * n1x + W1[2] * n1y;
+    float w00 = W0[0], w01 = qn<AX>(W0, 1, sn);
+    float w10 = W1[0], w11 = qn<AX>(W1, 1, sn);
 
     // dry/wet treatment (src/GPU/Arista_kernel.cu:301-321) with h2 = 0
     if ((w00 < eps) && (H1 - h11 > H0)) w11 = 0.0f;
@@ -385,15 +409,12 @@ __device__ __forceinline__ bool edge1(const K32& k, const float* W0, const float
 
     // tangential upwinding (src/GPU/Arista_kernel.cu:506-532)
     float ut1 = 0.0f;
-    if (!(fabsf(Fm0) < LH_EPSF)) {
-        ut1 = (Fm0 > 0.0f) ? (W0[2] * n1x - W0[1] * n1y) * s00 : (W1[2] * n1x - W1[1] * n1y) * s10;
-    }
-    float T1 = Fm0 * ut1;
+    if (!(fabsf(Fm0) < LH_EPSF)) ut1 = (Fm0 > 0.0f) ? W0[2 - AX] * s00 : W1[2 - AX] * s10;
     float* P = out.Fp;
     float* Mn = out.Fm;
-    Mn[0] = len * Fm0; Mn[1] = len * (Fm1 * n1x - T1 * n1y); Mn[2] = len * (Fm1 * n1y + T1 * n1x);
+    Mn[0] = len * Fm0;
     P[0] = -Mn[0];
-    P[1] = len * (Fp1 * n1x + T1 * n1y); P[2] = len * (Fp1 * n1y - T1 * n1x);
+    rotate_out<AX>(Mn, P, Fm1, Fp1, Fm0 * ut1, len, len * sn);
 
     // positivity with the partial accumulators (src/GPU/Arista_kernel.cu:554-609)
     const float area = k.area, dt = k.dtf, bq = k.bq;

@@ -362,15 +362,15 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
             if (has1) a1h1 = sA[0][r0 + 1][c];
             EdgeFlux F;
             if (__all(no_sediment(W0, W1, h20, h21))) {
-                if (edge1(kc, W0, W1, h10, h11, H0, H1, 0.0f, n1y, kc.len_y, a0h1, a1h1, interior, F)) {
+                if (edge1<1>(kc, W0, W1, h10, h11, H0, H1, n1y, kc.len_y, a0h1, a1h1, interior, F)) {
                     if (has0) accumulate<3>(sA, r0, c, F.Fm, F.lam);
                     if (has1) accumulate<3>(sA, r0 + 1, c, F.Fp, F.lam);
                 }
             } else {
                 const float a0h2 = has0 ? sA[3][r0][c] : 0.0f;
                 if (has1) a1h2 = sA[3][r0 + 1][c];
-                if (edge(kc, W0, W1, h10, h20, h11, h21, H0, H1, 0.0f, n1y, kc.len_y, a0h1, a0h2, a1h1, a1h2,
-                         interior, F)) {
+                if (edge<1>(kc, W0, W1, h10, h20, h11, h21, H0, H1, n1y, kc.len_y, a0h1, a0h2, a1h1, a1h2,
+                            interior, F)) {
                     if (has0) accumulate<6>(sA, r0, c, F.Fm, F.lam);
                     if (has1) accumulate<6>(sA, r0 + 1, c, F.Fp, F.lam);
                 }
@@ -428,15 +428,15 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
             if (interior) a1h1 = sA[0][r][c0 + 1];
             EdgeFlux F;
             if (__all(no_sediment(W0, W1, h20, h21))) {
-                if (edge1(kc, W0, W1, h10, h11, H0, H1, n1x, 0.0f, kc.len_x, a0h1, a1h1, interior, F)) {
+                if (edge1<0>(kc, W0, W1, h10, h11, H0, H1, n1x, kc.len_x, a0h1, a1h1, interior, F)) {
                     accumulate<3>(sA, r, c0, F.Fm, F.lam);
                     if (interior) accumulate<3>(sA, r, c0 + 1, F.Fp, F.lam);
                 }
             } else {
                 const float a0h2 = sA[3][r][c0];
                 if (interior) a1h2 = sA[3][r][c0 + 1];
-                if (edge(kc, W0, W1, h10, h20, h11, h21, H0, H1, n1x, 0.0f, kc.len_x, a0h1, a0h2, a1h1, a1h2,
-                         interior, F)) {
+                if (edge<0>(kc, W0, W1, h10, h20, h11, h21, H0, H1, n1x, kc.len_x, a0h1, a0h2, a1h1, a1h2,
+                            interior, F)) {
                     accumulate<6>(sA, r, c0, F.Fm, F.lam);
                     if (interior) accumulate<6>(sA, r, c0 + 1, F.Fp, F.lam);
                 }
@@ -701,7 +701,7 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
             const float a0h1 = has0 ? sA[0][r0][c] : 0.0f;
             if (has1) a1h1 = sA[0][r0 + 1][c];
             EdgeFlux F;
-            if (edge1(kc, W0, W1, h10, h11, H0, H1, 0.0f, n1y, kc.len_y, a0h1, a1h1, interior, F)) {
+            if (edge1<1>(kc, W0, W1, h10, h11, H0, H1, n1y, kc.len_y, a0h1, a1h1, interior, F)) {
                 if (has0) acc(r0, c, F.Fm, F.lam);
                 if (has1) acc(r0 + 1, c, F.Fp, F.lam);
             }
@@ -749,7 +749,7 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
             const float a0h1 = sA[0][r][c0];
             if (interior) a1h1 = sA[0][r][c0 + 1];
             EdgeFlux F;
-            if (edge1(kc, W0, W1, h10, h11, H0, H1, n1x, 0.0f, kc.len_x, a0h1, a1h1, interior, F)) {
+            if (edge1<0>(kc, W0, W1, h10, h11, H0, H1, n1x, kc.len_x, a0h1, a1h1, interior, F)) {
                 acc(r, c0, F.Fm, F.lam);
                 if (interior) acc(r, c0 + 1, F.Fp, F.lam);
             }

@@ -4,6 +4,9 @@
 //    on random edges whose two cells carry water only: wet, thin, dry and
 //    near-dry depths, steps in the bottom, interior and ghost (wall / open)
 //    edges in x and y;
+//  * an x-edge and the y-edge of the same states rotated by 90 degrees give
+//    rotated fluxes of equal values (edge() with and without sediment,
+//    edge1(), both normal signs);
 //  * cell_update1() must give cell_update()'s layer-1 state and cell_update()
 //    a zero layer-2 state, on random water-only cells with zero layer-2 sums
 //    (both FP32 checks under the Coulomb and the Pouliquen law);
@@ -78,8 +81,15 @@ int main(int argc, char** argv)
         }
         const float a0 = (float)((U(rng) - 0.5) * 100), a1 = (float)((U(rng) - 0.5) * 100);
         EdgeFlux A, B;
-        const bool wa = edge(k, W0, W1, h10, 0.0, h11, 0.0, H0, H1, n1x, n1y, len, a0, 0.0f, a1, 0.0f, interior, A);
-        const bool wb = edge1(k, W0, W1, h10, h11, H0, H1, n1x, n1y, len, a0, a1, interior, B);
+        const float sn = n1x + n1y;
+        bool wa, wb;
+        if (n1x != 0) {
+            wa = edge<0>(k, W0, W1, h10, 0.0, h11, 0.0, H0, H1, sn, len, a0, 0.0f, a1, 0.0f, interior, A);
+            wb = edge1<0>(k, W0, W1, h10, h11, H0, H1, sn, len, a0, a1, interior, B);
+        } else {
+            wa = edge<1>(k, W0, W1, h10, 0.0, h11, 0.0, H0, H1, sn, len, a0, 0.0f, a1, 0.0f, interior, A);
+            wb = edge1<1>(k, W0, W1, h10, h11, H0, H1, sn, len, a0, a1, interior, B);
+        }
         bool ok = wa == wb;
         if (ok && wa) {
             ok = same(A.lam, B.lam);
@@ -93,6 +103,90 @@ int main(int argc, char** argv)
                         B.Fm[0], B.Fm[1], B.Fm[2], A.lam, B.lam);
     }
     std::printf("edges %ld wet %ld dry %ld mismatches %ld\n", iters, wet, dry, bad);
+
+    // Rotation by 90 degrees, q -> (-qy, qx): the x-edge (normal (sn, 0)) on
+    // two states and the y-edge (normal (0, sn)) on the rotated states give
+    // the rotated fluxes, (Fx, Fy) -> (-Fy, Fx), with equal values; both edge
+    // forms, both layers (edge()), interior and ghost edges, both signs.
+    long rbad = 0, rwet = 0;
+    for (long it = 0; it < iters; it++) {
+        K32 k{};
+        k.eps = 5e-3f;
+        k.eps4 = k.eps * k.eps * k.eps * k.eps;
+        k.g = 9.81f;
+        k.r = (float)(0.45 + 0.15 * U(rng));
+        k.gp = (float)(9.81 * (1.0 - k.r));
+        k.igp = (float)(1.0 / (9.81 * (1.0 - k.r)));
+        k.mu = 0.17f;
+        k.peso = 1.0f;
+        k.dtf = (float)(0.05 + 0.3 * U(rng));
+        k.area = 3091.0f;
+        k.len_x = k.len_y = 55.6f;
+        k.ten_eps = 10.0f / k.eps;
+        k.idt_eps = (float)(1.0 / ((double)k.dtf + (double)LH_EPSF));
+        k.bq = (float)(k.dtf / (double)k.area);
+        k.pouliquen = U(rng) < 0.3;
+        k.tan1 = 0.15f; k.tan2 = 0.25f; k.tan3 = 0.17f; k.tan4 = 0.3f; k.LoverH = 1.0f;
+        const double H0 = 400 * U(rng);
+        double H1 = std::max(0.0, H0 + (U(rng) < 0.3 ? 0.0 : 20 * (U(rng) - 0.5)));
+        auto depth = [&](double H) {
+            const double r = U(rng);
+            return r < 0.15 ? 0.0 : r < 0.3 ? 0.01 * U(rng) : r < 0.4 ? 1e-4 * U(rng) : H * U(rng);
+        };
+        const bool sed = U(rng) < 0.5;
+        double h10 = depth(H0), h11 = depth(H1), h20 = sed ? depth(H0) : 0.0, h21 = sed ? depth(H1) : 0.0;
+        float W0[6] = {(float)h10, 0, 0, (float)h20, 0, 0}, W1[6] = {(float)h11, 0, 0, (float)h21, 0, 0};
+        for (int f : {1, 2}) {
+            W0[f] = (float)((U(rng) - 0.5) * 20 * (h10 > 0));
+            W1[f] = (float)((U(rng) - 0.5) * 20 * (h11 > 0));
+            W0[f + 3] = (float)((U(rng) - 0.5) * 4 * (h20 > 0));
+            W1[f + 3] = (float)((U(rng) - 0.5) * 4 * (h21 > 0));
+        }
+        const bool interior = U(rng) < 0.7;
+        const float sn = U(rng) < 0.5 ? 1.0f : -1.0f;
+        if (!interior) {
+            const float bd = U(rng) < 0.5 ? -1.0f : 1.0f;
+            for (int f = 0; f < 6; f++) W1[f] = W0[f];
+            W1[1] *= bd;
+            W1[4] *= bd;
+            h11 = h10;
+            h21 = h20;
+            H1 = H0;
+        }
+        float R0[6], R1[6];   // rotated states
+        for (int l = 0; l < 2; l++) {
+            R0[3 * l] = W0[3 * l]; R0[3 * l + 1] = -W0[3 * l + 2]; R0[3 * l + 2] = W0[3 * l + 1];
+            R1[3 * l] = W1[3 * l]; R1[3 * l + 1] = -W1[3 * l + 2]; R1[3 * l + 2] = W1[3 * l + 1];
+        }
+        const float a0 = (float)((U(rng) - 0.5) * 100), a1 = (float)((U(rng) - 0.5) * 100);
+        const float a2 = (float)((U(rng) - 0.5) * 100), a3 = (float)((U(rng) - 0.5) * 100);
+        auto rotated = [](const EdgeFlux& X, const EdgeFlux& Y, int nf) {
+            bool ok = X.lam == Y.lam;
+            for (int l = 0; l < nf / 3; l++) {
+                const int b = 3 * l;
+                ok = ok && X.Fm[b] == Y.Fm[b] && X.Fp[b] == Y.Fp[b];
+                ok = ok && Y.Fm[b + 1] == -X.Fm[b + 2] && Y.Fm[b + 2] == X.Fm[b + 1];
+                ok = ok && Y.Fp[b + 1] == -X.Fp[b + 2] && Y.Fp[b + 2] == X.Fp[b + 1];
+            }
+            return ok;
+        };
+        EdgeFlux X, Y;
+        const bool wx = edge<0>(k, W0, W1, h10, h20, h11, h21, H0, H1, sn, 55.6f, a0, a2, a1, a3, interior, X);
+        const bool wy = edge<1>(k, R0, R1, h10, h20, h11, h21, H0, H1, sn, 55.6f, a0, a2, a1, a3, interior, Y);
+        bool ok = wx == wy && (!wx || rotated(X, Y, 6));
+        if (!sed) {
+            EdgeFlux X1, Y1;
+            const bool vx = edge1<0>(k, W0, W1, h10, h11, H0, H1, sn, 55.6f, a0, a1, interior, X1);
+            const bool vy = edge1<1>(k, R0, R1, h10, h11, H0, H1, sn, 55.6f, a0, a1, interior, Y1);
+            ok = ok && vx == vy && (!vx || rotated(X1, Y1, 3));
+        }
+        rwet += wx;
+        if (!ok && ++rbad <= 5)
+            std::printf("rotation mismatch at %ld: x Fm %g %g %g / y Fm %g %g %g\n", it, X.Fm[0], X.Fm[1], X.Fm[2],
+                        Y.Fm[0], Y.Fm[1], Y.Fm[2]);
+    }
+    std::printf("rotated edges %ld wet %ld mismatches %ld\n", iters, rwet, rbad);
+    bad += rbad;
 
     long cbad = 0;
     for (long it = 0; it < iters; it++) {

@@ -7,8 +7,12 @@ with the terms that such edges and cells make zero removed.
 tests/native/edge1_check.cpp compiles the device functions for the host (g++,
 no FMA contraction, so both forms round the same operations) and compares
 them on random inputs: layer-1 results equal, layer-2 results zero.  The
-FP64 fused kernel's edge1() and cell_update1() are checked the same way against
-the FP64 edge() and cell_update().
+edges take axis-aligned normals (x-edge (sn, 0), y-edge (0, sn)); an x-edge
+and the y-edge of the same states rotated by 90 degrees must give the
+rotated fluxes with equal values, for edge() (with sediment too) and edge1()
+and both normal signs (a sign or component slip in one axis's projection or
+rotation fails it).  The FP64 fused kernel's edge1() and cell_update1() are
+checked the same way against the FP64 edge() and cell_update().
 tests/native/adjoint_ns_check.cpp does the same for the generated adjoint
 functions adj_edge_ns / adj_cell_ns (values and the adjoints of every input,
 FP32 and FP64).
@@ -31,9 +35,11 @@ def test_edge1_equals_edge_without_sediment(tmp_path):
     print(out.stdout)
     assert out.returncode == 0, out.stdout
     assert "edges" in out.stdout and "cells" in out.stdout
-    assert out.stdout.count("mismatches 0") == 4
+    assert out.stdout.count("mismatches 0") == 5
     wet = int(out.stdout.split("wet ")[1].split()[0])
     assert wet > 300000
+    rotated = int(out.stdout.split("rotated edges")[1].split("wet ")[1].split()[0])
+    assert rotated > 300000
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
