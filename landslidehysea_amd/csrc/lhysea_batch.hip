@@ -202,7 +202,7 @@ int prepare(lhysea_batch* b, bool with_outputs)
 int launch_direct(lhysea_batch* b, int nsteps)
 {
     bool buf = true;
-    for (const Scen& sc : b->sc) buf = buf && f32c_buf_fits(sc.N);
+    for (const Scen& sc : b->sc) buf = buf && state_buf_fits(sc.N);
     for (int n = 0; n < nsteps; n++) {
         hipError_t e = f32c_batch_step(b->tile, b->d_sd, (int)b->sc.size(), b->d_tab, b->nblocks, buf, b->stream);
         if (e != hipSuccess) return bhip_fail(e, "batch step launch");

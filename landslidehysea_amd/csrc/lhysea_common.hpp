@@ -40,6 +40,15 @@ struct Consts {
     ConstsF32 f;
 };
 
+// Raw buffer resources on gfx9 targets (gfx950 among them): word 3 selects
+// the 32-bit data format, no swizzle, no index stride.  The sediment-free
+// step kernels address a state of N cells through one resource with a
+// 32-bit byte offset per cell and the plane offset in a scalar register
+// while its 48 bytes per cell (12 FP32 or 6 FP64 planes) stay below 4 GiB
+// (N < 89,478,486, about 9459 x 9459); larger grids take 64-bit addresses.
+constexpr int BUF_DWORD3 = 0x00020000;
+__host__ __device__ inline bool state_buf_fits(size_t N) { return N < ((size_t)1 << 32) / 48; }
+
 // Device-side step control block (one per simulation).
 // maxA[] holds max_i A_i bit patterns (A_i >= 0, so the IEEE bit order is the
 // numeric order and an integer atomicMax is an exact float max).

@@ -187,10 +187,6 @@ __device__ __forceinline__ void accumulate(float (&sA)[7][TY][CW], int r, int c,
     sA[6][r][c] += lam;
 }
 
-// word 3 of a raw buffer resource on gfx9 targets (gfx950 among them):
-// 32-bit data format, no swizzle, no index stride
-constexpr int BUF_DWORD3 = 0x00020000;
-
 constexpr int FTX = 62, FTY = 14, FNT = 512;
 // general sub-tiles: width, threads, and the fixed grid walking them (four
 // workgroups per CU of an MI355X)
@@ -544,7 +540,7 @@ __device__ __forceinline__ double tile_step(const Consts& k, const float* __rest
 // BUF: the state and depth are read and written with buffer instructions
 // (plane offset in a scalar register, a 32-bit byte offset per cell), so that
 // no vector instruction forms an address; valid while the 12 FP32 planes stay
-// below 4 GiB (buf_fits).  Otherwise 64-bit addresses.
+// below 4 GiB (state_buf_fits).  Otherwise 64-bit addresses.
 template <int TX, int TY, int NT, bool BUF = false>
 __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __restrict__ W, float* __restrict__ Wn,
                                                const double* __restrict__ H, double dt, double tnow,
@@ -1265,7 +1261,7 @@ hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const dou
     hipError_t e = a.ev0 ? hipEventRecord(a.ev0, s) : hipSuccess;
     if (e != hipSuccess) return e;
     if (a.split) {
-        if (f32c_buf_fits((size_t)k.nx * k.ny))
+        if (state_buf_fits((size_t)k.nx * k.ny))
             hipLaunchKernelGGL((f32::k_fused_bare<f32::FTX, f32::FTY, f32::FNT, true>), dim3((unsigned)nt),
                                dim3(f32::FNT), 0, s, k, W, Wn, H, a, tc);
         else
@@ -1282,11 +1278,6 @@ hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const dou
     }
     return hipGetLastError();
 }
-
-// The sediment-free kernels address the state with buffer instructions when
-// its 12 FP32 planes of N cells stay below 4 GiB (N < 89,478,486, about
-// 9459 x 9459); larger grids take 64-bit addresses.
-bool f32c_buf_fits(size_t N) { return N < ((size_t)1 << 32) / 48; }
 
 // measured on an MI355X (single slide scenarios, step kernels with and without
 // the split): FP32C 1024^2 (1238 sediment-free tiles) 47.2 vs 47.3 us, 1536^2

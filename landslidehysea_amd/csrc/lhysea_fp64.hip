@@ -15,6 +15,7 @@
 // one 64-bit atomicMax per workgroup; a one-wave finalize launch turns the
 // max into the lagged dt of the next step on the device (no host sync).
 #pragma clang fp contract(off)
+#include <type_traits>
 #include "lhysea_internal.hpp"
 #include "lhysea_fp64.hpp"
 #include "lhysea_tiles.hpp"
@@ -215,7 +216,9 @@ __global__ void __launch_bounds__(256) k_edges(Consts k, const double* __restric
     const double a0h1 = acc[c0], a0h2 = acc[3 * N + c0];
     const double a1h1 = interior ? acc[c1] : 0.0, a1h2 = interior ? acc[3 * N + c1] : 0.0;
     EdgeFlux F;
-    if (edge(k, W0, W1, H0, H1, n1x, n1y, len, dt, a0h1, a0h2, a1h1, a1h2, interior, F)) {
+    const bool wet = (cls < 2) ? edge<1>(k, W0, W1, H0, H1, n1y, len, dt, a0h1, a0h2, a1h1, a1h2, interior, F)
+                               : edge<0>(k, W0, W1, H0, H1, n1x, len, dt, a0h1, a0h2, a1h1, a1h2, interior, F);
+    if (wet) {
         for (int f = 0; f < 6; f++) acc[f * N + c0] -= F.Fm[f];
         acc[6 * N + c0] += F.lam;
         if (interior) {
@@ -367,14 +370,14 @@ __device__ __forceinline__ double tile_step64(const Consts& k, const double* __r
             if (has1) a1h1 = sA[0][r0 + 1][c];
             EdgeFlux F;
             if (__all(no_sediment(W0, W1))) {
-                if (edge1(k, W0, W1, H0, H1, 0.0, n1y, k.dx, dt, a0h1, a1h1, interior, F)) {
+                if (edge1<1>(k, W0, W1, H0, H1, n1y, k.dx, dt, a0h1, a1h1, interior, F)) {
                     if (has0) accumulate<3>(sA, r0, c, F.Fm, F.lam);
                     if (has1) accumulate<3>(sA, r0 + 1, c, F.Fp, F.lam);
                 }
             } else {
                 const double a0h2 = has0 ? sA[3][r0][c] : 0.0;
                 if (has1) a1h2 = sA[3][r0 + 1][c];
-                if (edge(k, W0, W1, H0, H1, 0.0, n1y, k.dx, dt, a0h1, a0h2, a1h1, a1h2, interior, F)) {
+                if (edge<1>(k, W0, W1, H0, H1, n1y, k.dx, dt, a0h1, a0h2, a1h1, a1h2, interior, F)) {
                     if (has0) accumulate<6>(sA, r0, c, F.Fm, F.lam);
                     if (has1) accumulate<6>(sA, r0 + 1, c, F.Fp, F.lam);
                 }
@@ -418,14 +421,14 @@ __device__ __forceinline__ double tile_step64(const Consts& k, const double* __r
             if (interior) a1h1 = sA[0][r][c0 + 1];
             EdgeFlux F;
             if (__all(no_sediment(W0, W1))) {
-                if (edge1(k, W0, W1, H0, H1, n1x, 0.0, k.dy, dt, a0h1, a1h1, interior, F)) {
+                if (edge1<0>(k, W0, W1, H0, H1, n1x, k.dy, dt, a0h1, a1h1, interior, F)) {
                     accumulate<3>(sA, r, c0, F.Fm, F.lam);
                     if (interior) accumulate<3>(sA, r, c0 + 1, F.Fp, F.lam);
                 }
             } else {
                 const double a0h2 = sA[3][r][c0];
                 if (interior) a1h2 = sA[3][r][c0 + 1];
-                if (edge(k, W0, W1, H0, H1, n1x, 0.0, k.dy, dt, a0h1, a0h2, a1h1, a1h2, interior, F)) {
+                if (edge<0>(k, W0, W1, H0, H1, n1x, k.dy, dt, a0h1, a0h2, a1h1, a1h2, interior, F)) {
                     accumulate<6>(sA, r, c0, F.Fm, F.lam);
                     if (interior) accumulate<6>(sA, r, c0 + 1, F.Fp, F.lam);
                 }
@@ -503,19 +506,52 @@ constexpr int GGRID = 1024;
 // layer-2 planes there already, and only the layer-1 planes are written.
 // hc (optional): control block whose health word records the cell updates'
 // health bits as those of step s.
-template <int TX, int TY, int NT>
+// BUF: the state and depth are read and written with buffer instructions
+// (plane offset in a scalar register, a 32-bit byte offset per cell), valid
+// while state_buf_fits; otherwise 64-bit addresses.
+template <int TX, int TY, int NT, bool BUF = false>
 __device__ __forceinline__ bool tile_step_bare64(const Consts& k, const double* __restrict__ W,
                                                  double* __restrict__ Wn, const double* __restrict__ H, double dt,
                                                  double tnow, double* __restrict__ emax, int i0, int j0,
                                                  const int* cls, int s, int skip_from, GaugeOut go, double& Amax,
                                                  Ctl* hc = nullptr)
 {
+    using Idx = typename std::conditional<BUF, unsigned, size_t>::type;
     constexpr int CW = TX + 2, RW = TY + 2;
     __shared__ double sW[4][RW][CW];   // h1, q1x, q1y, H
     __shared__ double sA[4][TY][CW];   // dh1, dq1x, dq1y, wave-speed sum
     const int nx = k.nx, ny = k.ny;
     const size_t N = (size_t)nx * ny;
     const int tid = threadIdx.x;
+    // FP64 plane p at cell g of the old / new state, depth at cell g
+    [[maybe_unused]] __amdgpu_buffer_rsrc_t rW, rWn, rH;
+    [[maybe_unused]] unsigned pb = 0u;
+    if constexpr (BUF) {
+        pb = (unsigned)N * 8u;
+        rW = __builtin_amdgcn_make_buffer_rsrc((void*)W, (short)0, (int)(6u * pb), BUF_DWORD3);
+        rWn = __builtin_amdgcn_make_buffer_rsrc((void*)Wn, (short)0, (int)(6u * pb), BUF_DWORD3);
+        rH = __builtin_amdgcn_make_buffer_rsrc((void*)H, (short)0, (int)pb, BUF_DWORD3);
+    }
+    auto ldW = [&](int p, Idx g) -> double {
+        if constexpr (BUF)
+            return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rW, g * 8u, p * pb, 0));
+        else
+            return W[p * N + g];
+    };
+    auto ldH = [&](Idx g) -> double {
+        if constexpr (BUF)
+            return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rH, g * 8u, 0, 0));
+        else
+            return H[g];
+    };
+    auto stW = [&](int p, Idx g, double v) {
+        if constexpr (BUF) {
+            using raw64 = decltype(__builtin_amdgcn_raw_buffer_load_b64(rW, 0u, 0u, 0));
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(raw64, v), rWn, g * 8u, p * pb, 0);
+        } else {
+            Wn[p * N + g] = v;
+        }
+    };
 
     constexpr int ST_IT = (RW * CW + NT - 1) / NT;
     double v[ST_IT][4];
@@ -527,11 +563,11 @@ __device__ __forceinline__ bool tile_step_bare64(const Consts& k, const double* 
         const int gj = j0 + r - 1, gi = i0 + c - 1;
         in[it] = e < RW * CW && gj >= 0 && gj < ny && gi >= 0 && gi < nx;
         if (in[it]) {
-            const size_t g = (size_t)gj * nx + gi;
-            v[it][0] = W[g];
-            v[it][1] = W[N + g];
-            v[it][2] = W[2 * N + g];
-            v[it][3] = H[g];
+            const Idx g = (Idx)gj * nx + gi;
+            v[it][0] = ldW(0, g);
+            v[it][1] = ldW(1, g);
+            v[it][2] = ldW(2, g);
+            v[it][3] = ldH(g);
         }
     }
     const int cv = *cls;
@@ -590,7 +626,7 @@ __device__ __forceinline__ bool tile_step_bare64(const Consts& k, const double* 
             const double a0h1 = has0 ? sA[0][r0][c] : 0.0;
             if (has1) a1h1 = sA[0][r0 + 1][c];
             EdgeFlux F;
-            if (edge1(k, W0, W1, H0, H1, 0.0, n1y, k.dx, dt, a0h1, a1h1, interior, F)) {
+            if (edge1<1>(k, W0, W1, H0, H1, n1y, k.dx, dt, a0h1, a1h1, interior, F)) {
                 if (has0) acc(r0, c, F.Fm, F.lam);
                 if (has1) acc(r0 + 1, c, F.Fp, F.lam);
             }
@@ -630,7 +666,7 @@ __device__ __forceinline__ bool tile_step_bare64(const Consts& k, const double* 
             const double a0h1 = sA[0][r][c0];
             if (interior) a1h1 = sA[0][r][c0 + 1];
             EdgeFlux F;
-            if (edge1(k, W0, W1, H0, H1, n1x, 0.0, k.dy, dt, a0h1, a1h1, interior, F)) {
+            if (edge1<0>(k, W0, W1, H0, H1, n1x, k.dy, dt, a0h1, a1h1, interior, F)) {
                 acc(r, c0, F.Fm, F.lam);
                 if (interior) acc(r, c0 + 1, F.Fp, F.lam);
             }
@@ -645,7 +681,7 @@ __device__ __forceinline__ bool tile_step_bare64(const Consts& k, const double* 
         const int r = e / TX, c = e % TX;
         const int gj = j0 + r, gi = i0 + c;
         if (gj >= ny || gi >= nx) continue;
-        const size_t g = (size_t)gj * nx + gi;
+        const Idx g = (Idx)gj * nx + gi;
         double o[6] = {sW[0][r + 1][c + 1], sW[1][r + 1][c + 1], sW[2][r + 1][c + 1], 0.0, 0.0, 0.0};
         double ac[6] = {sA[0][r][c + 1], sA[1][r][c + 1], sA[2][r][c + 1], 0.0, 0.0, 0.0};
         double n[6];
@@ -659,10 +695,10 @@ __device__ __forceinline__ bool tile_step_bare64(const Consts& k, const double* 
         }
         cell_update1(k, o, ac, dt, n, &hb);
 #pragma unroll
-        for (int f = 0; f < 3; f++) Wn[f * N + g] = n[f];
+        for (int f = 0; f < 3; f++) stW(f, g, n[f]);
         if (zero2) {
 #pragma unroll
-            for (int f = 3; f < 6; f++) Wn[f * N + g] = n[f];
+            for (int f = 3; f < 6; f++) stW(f, g, n[f]);
         }
         if (go.out) sW[0][r + 1][c + 1] = n[0];   // the new h1 (read by no other thread now)
     }
@@ -719,7 +755,7 @@ __global__ void __launch_bounds__(NT) k_recompute(Consts k, const double* __rest
                                   GaugeOut{nullptr, 0, nullptr}, nullptr, 0, 0, nullptr, 0, P, ACC);
 }
 
-template <int TX, int TY, int NT>
+template <int TX, int TY, int NT, bool BUF>
 __global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const double* __restrict__ W, double* __restrict__ Wn,
                                                    const double* __restrict__ H, StepArgs a, TileCls tc)
 {
@@ -730,7 +766,7 @@ __global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const double* __res
     const int s = a.ctl->step;
     const GaugeOut go{a.gauges, a.ng, (a.gseries && s < a.gcap) ? a.gseries + (size_t)s * a.ng : nullptr};
     double Amax;
-    if (!tile_step_bare64<TX, TY, NT>(k, W, Wn, H, step_dt(a.ctl, a.dt_seq), a.ctl->t, a.emax, tx * TX, ty * TY,
+    if (!tile_step_bare64<TX, TY, NT, BUF>(k, W, Wn, H, step_dt(a.ctl, a.dt_seq), a.ctl->t, a.emax, tx * TX, ty * TY,
                                       tc.cls + (s & 1) * nb + b, s, a.wn_prev ? a.ctl->cls_from : 0x7fffffff, go,
                                       Amax, a.ctl))
         return;
@@ -910,8 +946,12 @@ hipError_t f64_step_fused(const Consts& k, const double* W, double* Wn, const do
     hipError_t e = a.ev0 ? hipEventRecord(a.ev0, s) : hipSuccess;
     if (e != hipSuccess) return e;
     if (a.split) {
-        hipLaunchKernelGGL((f64::k_fused_bare<f64::FTX, f64::FTY, f64::FNT>), dim3((unsigned)nt), dim3(f64::FNT), 0,
-                           s, k, W, Wn, H, a, tc);
+        if (state_buf_fits((size_t)k.nx * k.ny))
+            hipLaunchKernelGGL((f64::k_fused_bare<f64::FTX, f64::FTY, f64::FNT, true>), dim3((unsigned)nt),
+                               dim3(f64::FNT), 0, s, k, W, Wn, H, a, tc);
+        else
+            hipLaunchKernelGGL((f64::k_fused_bare<f64::FTX, f64::FTY, f64::FNT, false>), dim3((unsigned)nt),
+                               dim3(f64::FNT), 0, s, k, W, Wn, H, a, tc);
         hipLaunchKernelGGL((f64::k_fused_gen<f64::FTX, f64::FTY, f64::FNT>), dim3(f64::GGRID), dim3(f64::FNT), 0, s,
                            k, W, Wn, H, a, tc);
         if (a.ev1 && (e = hipEventRecord(a.ev1, s)) != hipSuccess) return e;

@@ -86,19 +86,46 @@ struct EdgeFlux {
     double lam;
 };
 
+// Edge normals are axis-aligned: AX 0 is an x-edge with normal (sn, 0), AX 1
+// a y-edge with normal (0, sn), sn = +-1.  With the normal's zero component
+// dropped the projections and rotations equal the general forms (q . n,
+// q . t with t = (-ny, nx), and back) exactly, up to the sign of zero
+// results; the tangential flux F[h] u.t does not depend on sn (u.t is odd in
+// the tangential discharge, and the two sign flips cancel).
+// Normal component of the discharge (qx, qy) = (W[f], W[f + 1]).
+template <int AX>
+__device__ __forceinline__ double qn(const double* W, int f, double sn)
+{
+    return sn * W[f + AX];
+}
+
+// The edge's fluxes in the x/y frame from the normal fluxes Fm, Fp and the
+// tangential flux Ft (per unit length), lsn = len * sn:
+// n-frame (F, Ft) -> (F nx - Ft ny, F ny + Ft nx), and F+ rotated with -Ft.
+template <int AX>
+__device__ __forceinline__ void rotate_out(double* Mn, double* P, double Fm, double Fp, double Ft, double len,
+                                           double lsn)
+{
+    const double lt = len * Ft;
+    Mn[1 + AX] = lsn * Fm;
+    Mn[2 - AX] = lt;
+    P[1 + AX] = lsn * Fp;
+    P[2 - AX] = -lt;
+}
+
 // One edge of the two-layer path-conservative PVM solver
 // (src/GPU/Arista_kernel.cu:324-645).  W0/W1 unrotated states; W1 is the
-// ghost when !interior.  (n1x,n1y) is the unit normal (entries 0 or +-1).
+// ghost when !interior.  Normal of axis AX with sign sn.
 // a0h1.. are the partial accumulators of h1/h2 read by the positivity limiter.
 // Returns false for a dry edge (no contribution at all).
+template <int AX>
 __device__ __forceinline__ bool edge(const Consts& k, const double* W0, const double* W1, double H0, double H1,
-                                     double n1x, double n1y, double len, double dt,
-                                     double a0h1, double a0h2, double a1h1, double a1h2, bool interior,
-                                     EdgeFlux& out)
+                                     double sn, double len, double dt, double a0h1, double a0h2, double a1h1,
+                                     double a1h2, bool interior, EdgeFlux& out)
 {
     const double r = k.r, g = k.g, eps = k.eps_h, area = k.area;
-    double w00 = W0[0], w01 = W0[1] * n1x + W0[2] * n1y, w02 = W0[3], w03 = W0[4] * n1x + W0[5] * n1y;
-    double w10 = W1[0], w11 = W1[1] * n1x + W1[2] * n1y, w12 = W1[3], w13 = W1[4] * n1x + W1[5] * n1y;
+    double w00 = W0[0], w01 = qn<AX>(W0, 1, sn), w02 = W0[3], w03 = qn<AX>(W0, 4, sn);
+    double w10 = W1[0], w11 = qn<AX>(W1, 1, sn), w12 = W1[3], w13 = qn<AX>(W1, 4, sn);
 
     // dry/wet treatment (src/GPU/Arista_kernel.cu:301-321)
     if ((w02 < eps) && (H1 - w12 > H0)) w13 = 0.0;
@@ -200,21 +227,23 @@ __device__ __forceinline__ bool edge(const Consts& k, const double* W0, const do
     double Fm0 = M0 + (FL0 - V0), Fm1 = M1 + (FL1 - V1), Fm2 = M2 + (FL2 - V2), Fm3 = M3 + (FL3 - V3);
     double Fp0 = M0 + (V0 - FR0), Fp1 = M1 + (V1 - FR1), Fp2 = M2 + (V2 - FR2), Fp3 = M3 + (V3 - FR3);
 
-    // tangential upwinding (src/GPU/Arista_kernel.cu:506-532)
-    double q0t = W0[2] * n1x - W0[1] * n1y, q1t = W1[2] * n1x - W1[1] * n1y;
-    double ut1 = (fabs(Fm0) < LH_EPS) ? 0.0 : ((Fm0 > 0) ? desing_u(w00, q0t, eps) : desing_u(w10, q1t, eps));
-    q0t = W0[5] * n1x - W0[4] * n1y;
-    q1t = W1[5] * n1x - W1[4] * n1y;
-    double ut2 = (fabs(Fm2) < LH_EPS) ? 0.0 : ((Fm2 > 0) ? desing_u(w02, q0t, eps) : desing_u(w12, q1t, eps));
+    // tangential upwinding (src/GPU/Arista_kernel.cu:506-532): the upwind
+    // cell's tangential velocity, up to the normal's sign (see rotate_out)
+    double ut1 = (fabs(Fm0) < LH_EPS) ? 0.0
+                                      : ((Fm0 > 0) ? desing_u(w00, W0[2 - AX], eps) : desing_u(w10, W1[2 - AX], eps));
+    double ut2 = (fabs(Fm2) < LH_EPS) ? 0.0
+                                      : ((Fm2 > 0) ? desing_u(w02, W0[5 - AX], eps) : desing_u(w12, W1[5 - AX], eps));
 
     // back to 6 components times the edge length (src/GPU/Arista_kernel.cu:534-552)
-    double T1 = Fm0 * ut1, T2 = Fm2 * ut2;
     double* P = out.Fp;
     double* Mn = out.Fm;
-    P[0] = len * Fp0; P[1] = len * (Fp1 * n1x + T1 * n1y); P[2] = len * (Fp1 * n1y - T1 * n1x);
-    Mn[0] = len * Fm0; Mn[1] = len * (Fm1 * n1x - T1 * n1y); Mn[2] = len * (Fm1 * n1y + T1 * n1x);
-    P[3] = len * Fp2; P[4] = len * (Fp3 * n1x + T2 * n1y); P[5] = len * (Fp3 * n1y - T2 * n1x);
-    Mn[3] = len * Fm2; Mn[4] = len * (Fm3 * n1x - T2 * n1y); Mn[5] = len * (Fm3 * n1y + T2 * n1x);
+    const double lsn = len * sn;
+    P[0] = len * Fp0;
+    Mn[0] = len * Fm0;
+    P[3] = len * Fp2;
+    Mn[3] = len * Fm2;
+    rotate_out<AX>(Mn, P, Fm1, Fp1, Fm0 * ut1, len, lsn);
+    rotate_out<AX>(Mn + 3, P + 3, Fm3, Fp3, Fm2 * ut2, len, lsn);
 
     // positivity with the partial accumulators (src/GPU/Arista_kernel.cu:554-609)
     const double factor = 1.0 * k.beta;
@@ -269,13 +298,14 @@ __device__ __forceinline__ void net_zero_pair(double& Dx, double& Dw, double* au
 // Fp[3..5] unset).  With h2ij = 0 the Coulomb flag is false for either
 // friction law, deta2 = 0, the internal eigenvalue pair is (0, 0) and the PVM
 // takes its two-wave or Rusanov form.
+template <int AX>
 __device__ __forceinline__ bool edge1(const Consts& k, const double* W0, const double* W1, double H0, double H1,
-                                      double n1x, double n1y, double len, double dt, double a0h1, double a1h1,
-                                      bool interior, EdgeFlux& out)
+                                      double sn, double len, double dt, double a0h1, double a1h1, bool interior,
+                                      EdgeFlux& out)
 {
     const double g = k.g, eps = k.eps_h, area = k.area;
-    double w00 = W0[0], w01 = W0[1] * n1x + W0[2] * n1y;
-    double w10 = W1[0], w11 = W1[1] * n1x + W1[2] * n1y;
+    double w00 = W0[0], w01 = qn<AX>(W0, 1, sn);
+    double w10 = W1[0], w11 = qn<AX>(W1, 1, sn);
 
     // dry/wet treatment (src/GPU/Arista_kernel.cu:301-321) with h2 = 0
     if ((w00 < eps) && (H1 - w10 > H0)) w11 = 0.0;
@@ -346,13 +376,13 @@ __device__ __forceinline__ bool edge1(const Consts& k, const double* W0, const d
     double Fp0 = M0 + (V0 - FR0), Fp1 = M1 + (V1 - FR1);
 
     // tangential upwinding (src/GPU/Arista_kernel.cu:506-532)
-    double q0t = W0[2] * n1x - W0[1] * n1y, q1t = W1[2] * n1x - W1[1] * n1y;
-    double ut1 = (fabs(Fm0) < LH_EPS) ? 0.0 : ((Fm0 > 0) ? desing_u(w00, q0t, eps) : desing_u(w10, q1t, eps));
-    double T1 = Fm0 * ut1;
+    double ut1 = (fabs(Fm0) < LH_EPS) ? 0.0
+                                      : ((Fm0 > 0) ? desing_u(w00, W0[2 - AX], eps) : desing_u(w10, W1[2 - AX], eps));
     double* P = out.Fp;
     double* Mn = out.Fm;
-    P[0] = len * Fp0; P[1] = len * (Fp1 * n1x + T1 * n1y); P[2] = len * (Fp1 * n1y - T1 * n1x);
-    Mn[0] = len * Fm0; Mn[1] = len * (Fm1 * n1x - T1 * n1y); Mn[2] = len * (Fm1 * n1y + T1 * n1x);
+    P[0] = len * Fp0;
+    Mn[0] = len * Fm0;
+    rotate_out<AX>(Mn, P, Fm1, Fp1, Fm0 * ut1, len, len * sn);
 
     // positivity with the partial accumulators (src/GPU/Arista_kernel.cu:554-609)
     const double factor = 1.0 * k.beta;

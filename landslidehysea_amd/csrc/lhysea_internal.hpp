@@ -88,9 +88,6 @@ hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const dou
 int f32c_tiles(const Consts& k);
 // fewest sediment-free tiles for which the split step pays
 int f32c_split_min_bare();
-// the FP32C sediment-free kernels' state of N cells is addressable with
-// buffer instructions (12 FP32 planes below 4 GiB)
-bool f32c_buf_fits(size_t N);
 // class table of state W at step ctl->step (after an upload or a restore)
 hipError_t f32c_classify(const Consts& k, const float* W, const StepArgs& a, hipStream_t s);
 // partial h sums read by classes 2..4 (P, [3][2][N]) and flux sums (ACC, [6][N])
@@ -126,7 +123,7 @@ struct BatchTile {
 };
 constexpr int kNumBatchTiles = 4;
 extern const BatchTile kBatchTiles[kNumBatchTiles];
-// buf: every scenario's state passes f32c_buf_fits (buffer-addressed kernels)
+// buf: every scenario's state passes state_buf_fits (buffer-addressed kernels)
 hipError_t f32c_batch_step(int tile, const BatchScen* sd, int nscen, const unsigned* tab, int nblocks, bool buf,
                            hipStream_t s);
 // scenarios of a batch at most

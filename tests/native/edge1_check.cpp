@@ -12,7 +12,7 @@
 //    (both FP32 checks under the Coulomb and the Pouliquen law);
 //  * the FP64 edge1() must give the FP64 edge()'s layer-1 fluxes and wave
 //    speed, and the FP64 cell_update1() cell_update()'s state, for the
-//    Coulomb and the Pouliquen law.
+//    Coulomb and the Pouliquen law; the FP64 edges pass the rotation check.
 // Build: g++ -std=c++17 -O2 -ffp-contract=off -I tests/native edge1_check.cpp
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -273,8 +273,15 @@ int main(int argc, char** argv)
         }
         const double a0 = (U(rng) - 0.5) * 100, a1 = (U(rng) - 0.5) * 100;
         lhysea::f64::EdgeFlux A, B;
-        const bool wa = lhysea::f64::edge(k, W0, W1, H0, H1, n1x, n1y, len, dt, a0, 0.0, a1, 0.0, interior, A);
-        const bool wb = lhysea::f64::edge1(k, W0, W1, H0, H1, n1x, n1y, len, dt, a0, a1, interior, B);
+        const double sn = n1x + n1y;
+        bool wa, wb;
+        if (n1x != 0) {
+            wa = lhysea::f64::edge<0>(k, W0, W1, H0, H1, sn, len, dt, a0, 0.0, a1, 0.0, interior, A);
+            wb = lhysea::f64::edge1<0>(k, W0, W1, H0, H1, sn, len, dt, a0, a1, interior, B);
+        } else {
+            wa = lhysea::f64::edge<1>(k, W0, W1, H0, H1, sn, len, dt, a0, 0.0, a1, 0.0, interior, A);
+            wb = lhysea::f64::edge1<1>(k, W0, W1, H0, H1, sn, len, dt, a0, a1, interior, B);
+        }
         bool ok = wa == wb;
         if (ok && wa) {
             ok = A.lam == B.lam;
@@ -287,6 +294,79 @@ int main(int argc, char** argv)
                         A.Fm[2], B.Fm[0], B.Fm[1], B.Fm[2], A.lam, B.lam);
     }
     std::printf("fp64 edges %ld wet %ld mismatches %ld\n", iters, dwet, dbad);
+
+    // the FP64 edges under the 90-degree rotation, as the FP32 ones above
+    long drbad = 0, drwet = 0;
+    for (long it = 0; it < iters; it++) {
+        lhysea::Consts k{};
+        k.r = 0.45 + 0.15 * U(rng);
+        k.g = 9.81;
+        k.eps_h = 5e-3;
+        k.area = 3091.0;
+        k.mu = 0.17;
+        k.peso = U(rng) < 0.5 ? 1.0 : 0.9;
+        k.beta = 1.0;
+        k.friction_law = U(rng) < 0.3 ? 1 : 0;
+        k.tan1 = 0.1; k.tan2 = 0.3; k.tan3 = 0.12; k.tan4 = 0.35; k.LoverH = 1.0;
+        const double dt = 0.05 + 0.3 * U(rng);
+        const double H0 = 400 * U(rng);
+        double H1 = std::max(0.0, H0 + (U(rng) < 0.3 ? 0.0 : 20 * (U(rng) - 0.5)));
+        auto depth = [&](double H) {
+            const double r = U(rng);
+            return r < 0.15 ? 0.0 : r < 0.3 ? 0.01 * U(rng) : r < 0.4 ? 1e-4 * U(rng) : H * U(rng);
+        };
+        const bool sed = U(rng) < 0.5;
+        double W0[6] = {depth(H0), 0, 0, sed ? depth(H0) : 0.0, 0, 0};
+        double W1[6] = {depth(H1), 0, 0, sed ? depth(H1) : 0.0, 0, 0};
+        for (int f : {1, 2}) {
+            W0[f] = (U(rng) - 0.5) * 20 * (W0[0] > 0);
+            W1[f] = (U(rng) - 0.5) * 20 * (W1[0] > 0);
+            W0[f + 3] = (U(rng) - 0.5) * 4 * (W0[3] > 0);
+            W1[f + 3] = (U(rng) - 0.5) * 4 * (W1[3] > 0);
+        }
+        const bool interior = U(rng) < 0.7;
+        const double sn = U(rng) < 0.5 ? 1.0 : -1.0;
+        if (!interior) {
+            const double bd = U(rng) < 0.5 ? -1.0 : 1.0;
+            for (int f = 0; f < 6; f++) W1[f] = W0[f];
+            W1[1] *= bd;
+            W1[4] *= bd;
+            H1 = H0;
+        }
+        double R0[6], R1[6];
+        for (int l = 0; l < 2; l++) {
+            R0[3 * l] = W0[3 * l]; R0[3 * l + 1] = -W0[3 * l + 2]; R0[3 * l + 2] = W0[3 * l + 1];
+            R1[3 * l] = W1[3 * l]; R1[3 * l + 1] = -W1[3 * l + 2]; R1[3 * l + 2] = W1[3 * l + 1];
+        }
+        const double a0 = (U(rng) - 0.5) * 100, a1 = (U(rng) - 0.5) * 100;
+        const double a2 = (U(rng) - 0.5) * 100, a3 = (U(rng) - 0.5) * 100;
+        auto rotated = [](const lhysea::f64::EdgeFlux& X, const lhysea::f64::EdgeFlux& Y, int nf) {
+            bool ok = X.lam == Y.lam;
+            for (int l = 0; l < nf / 3; l++) {
+                const int b = 3 * l;
+                ok = ok && X.Fm[b] == Y.Fm[b] && X.Fp[b] == Y.Fp[b];
+                ok = ok && Y.Fm[b + 1] == -X.Fm[b + 2] && Y.Fm[b + 2] == X.Fm[b + 1];
+                ok = ok && Y.Fp[b + 1] == -X.Fp[b + 2] && Y.Fp[b + 2] == X.Fp[b + 1];
+            }
+            return ok;
+        };
+        lhysea::f64::EdgeFlux X, Y;
+        const bool wx = lhysea::f64::edge<0>(k, W0, W1, H0, H1, sn, 55.6, dt, a0, a2, a1, a3, interior, X);
+        const bool wy = lhysea::f64::edge<1>(k, R0, R1, H0, H1, sn, 55.6, dt, a0, a2, a1, a3, interior, Y);
+        bool ok = wx == wy && (!wx || rotated(X, Y, 6));
+        if (!sed) {
+            lhysea::f64::EdgeFlux X1, Y1;
+            const bool vx = lhysea::f64::edge1<0>(k, W0, W1, H0, H1, sn, 55.6, dt, a0, a1, interior, X1);
+            const bool vy = lhysea::f64::edge1<1>(k, R0, R1, H0, H1, sn, 55.6, dt, a0, a1, interior, Y1);
+            ok = ok && vx == vy && (!vx || rotated(X1, Y1, 3));
+        }
+        drwet += wx;
+        if (!ok && ++drbad <= 5)
+            std::printf("fp64 rotation mismatch at %ld: x Fm %g %g %g / y Fm %g %g %g\n", it, X.Fm[0], X.Fm[1],
+                        X.Fm[2], Y.Fm[0], Y.Fm[1], Y.Fm[2]);
+    }
+    std::printf("fp64 rotated edges %ld wet %ld mismatches %ld\n", iters, drwet, drbad);
+    dbad += drbad;
 
     long dcbad = 0;
     for (long it = 0; it < iters; it++) {

@@ -118,6 +118,8 @@ inline unsigned long long __builtin_amdgcn_raw_buffer_load_b64(__amdgpu_buffer_r
     unsigned long long v; std::memcpy(&v, emu_buf_at(r, voff, soff, 8), 8); return v; }
 inline void __builtin_amdgcn_raw_buffer_store_b32(unsigned v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, int) {
     std::memcpy(emu_buf_at(r, voff, soff, 4), &v, 4); }
+inline void __builtin_amdgcn_raw_buffer_store_b64(unsigned long long v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, int) {
+    std::memcpy(emu_buf_at(r, voff, soff, 8), &v, 8); }
 extern "C" float fmaf(float, float, float);
 inline float emu_fmaf(float a, float b, float c) { return ::fmaf(a, b, c); }
 #define __builtin_fmaf(a, b, c) emu_fmaf(a, b, c)

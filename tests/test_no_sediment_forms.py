@@ -12,7 +12,8 @@ and the y-edge of the same states rotated by 90 degrees must give the
 rotated fluxes with equal values, for edge() (with sediment too) and edge1()
 and both normal signs (a sign or component slip in one axis's projection or
 rotation fails it).  The FP64 fused kernel's edge1() and cell_update1() are
-checked the same way against the FP64 edge() and cell_update().
+checked the same way against the FP64 edge() and cell_update(), and the FP64
+edges pass the same rotation check.
 tests/native/adjoint_ns_check.cpp does the same for the generated adjoint
 functions adj_edge_ns / adj_cell_ns (values and the adjoints of every input,
 FP32 and FP64).
@@ -35,7 +36,7 @@ def test_edge1_equals_edge_without_sediment(tmp_path):
     print(out.stdout)
     assert out.returncode == 0, out.stdout
     assert "edges" in out.stdout and "cells" in out.stdout
-    assert out.stdout.count("mismatches 0") == 5
+    assert out.stdout.count("mismatches 0") == 6
     wet = int(out.stdout.split("wet ")[1].split()[0])
     assert wet > 300000
     rotated = int(out.stdout.split("rotated edges")[1].split("wet ")[1].split()[0])
