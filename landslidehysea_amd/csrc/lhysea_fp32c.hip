@@ -843,12 +843,17 @@ __global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const float* __rest
 // returned, so it is issued once that max has been performed, and the
 // closing workgroup reads the max with an atomic.  The sediment-free launch
 // ran before this one, so its maxima are in place.
+// The marks of a workgroup's last work item go out from its second wavefront
+// while the first one counts the workgroup done and maybe closes the step,
+// so their atomics' latencies overlap: the marks write the next step's class
+// table, list and count, which the close does not touch, and the next step's
+// launches start after this one has ended.
 template <int STX, int TY, int NT>
 __global__ void __launch_bounds__(NT, 3) k_fused_gen(Consts k, const float* __restrict__ W, float* __restrict__ Wn,
                                                   const double* __restrict__ H, StepArgs a, TileCls tc)
 {
     constexpr int NSUB = (FTX + STX - 1) / STX;
-    static_assert(TY == FTY && STX % 2 == 0, "sub-tiles of a class tile");
+    static_assert(TY == FTY && STX % 2 == 0 && NT >= 128, "sub-tiles of a class tile; two wavefronts");
     Ctl* ctl = a.ctl;
     // the step is the launch's own: a workgroup that starts after the last
     // participant closed it would otherwise read the next step's list
@@ -859,12 +864,14 @@ __global__ void __launch_bounds__(NT, 3) k_fused_gen(Consts k, const float* __re
     const unsigned part = (unsigned)max(min(n, (int)gridDim.x), 1);
     if (blockIdx.x >= part) return;
     const int nt = tc.tiles_x * tc.tiles_y;
+    const int* list = tc.list + (s & 1) * nt;
     const double dt = step_dt(ctl, a.dt_seq);
     const double tnow = ctl->t;
     const GaugeOut go{a.gauges, a.ng, (a.gseries && s < a.gcap) ? a.gseries + (size_t)s * a.ng : nullptr};
     double Amax = 0.0;
+    int mlast = 0, txl = 0, tyl = 0;
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
-        const int t = tc.list[(s & 1) * nt + i / NSUB], sub = i % NSUB;
+        const int t = list[i / NSUB], sub = i % NSUB;
         const int tx = t % tc.tiles_x, ty = t / tc.tiles_x;
         const int cx0 = tx * FTX, cy0 = ty * FTY;
         int m = 0;
@@ -872,7 +879,13 @@ __global__ void __launch_bounds__(NT, 3) k_fused_gen(Consts k, const float* __re
                                                  nullptr, cx0 + FTX, go, &m, cx0, cy0, ctl, s));
         // also orders this sub-tile's LDS reads before the next one's staging
         m = block_or(m);
-        if (m && threadIdx.x < 64) mark_general(tc, ctl, tx, ty, s + 1, m);
+        if (i + (int)gridDim.x < n) {
+            if (m && threadIdx.x < 64) mark_general(tc, ctl, tx, ty, s + 1, m);
+        } else {
+            mlast = m;
+            txl = tx;
+            tyl = ty;
+        }
     }
     // close of the step
     __shared__ double red[NT / 64];
@@ -880,6 +893,7 @@ __global__ void __launch_bounds__(NT, 3) k_fused_gen(Consts k, const float* __re
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
+    if (mlast && threadIdx.x >= 64 && threadIdx.x < 128) mark_general(tc, ctl, txl, tyl, s + 1, mlast);
     if (threadIdx.x == 0) {
         for (int w = 1; w < NT / 64; w++) v = fmax(v, red[w]);
         v = fabs(fmax(v, red[0]));   // sign bit clear also for a NaN from a corrupt state
