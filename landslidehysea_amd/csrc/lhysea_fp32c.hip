@@ -188,6 +188,13 @@ __device__ __forceinline__ void accumulate(float (&sA)[7][TY][CW], int r, int c,
 }
 
 constexpr int FTX = 62, FTY = 14, FNT = 512;
+// Scalar registers of the sediment-free kernels: a wave that needs more than
+// 80 (74 addressable plus the reserved ones) is allocated 96 on gfx950, and
+// at 96 a CU no longer holds four 512-thread workgroups (measured: 82% of the
+// resident wave-cycles, the 4096^2 step 289 -> 331 us).  The buffer-addressed
+// kernels fit in 80; the 64-bit addressed ones are held there by this cap,
+// which spills a few scalars to vector lanes.
+#define LH_BARE_SGPR_CAP __attribute__((amdgpu_num_sgpr(82)))
 // general sub-tiles: width, threads, and the fixed grid walking them (four
 // workgroups per CU of an MI355X)
 constexpr int GSTX = 16, GNT = 256, GGRID = 1024;
@@ -809,8 +816,8 @@ __device__ __forceinline__ bool tile_step_bare(const Consts& k, const float* __r
 // runs blocks x, x+8, ...; giving it one run of tiles keeps shared halo rows
 // in its L2).
 template <int TX, int TY, int NT, bool BUF>
-__global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const float* __restrict__ W, float* __restrict__ Wn,
-                                                   const double* __restrict__ H, StepArgs a, TileCls tc)
+__device__ __forceinline__ void fused_bare(const Consts& k, const float* __restrict__ W, float* __restrict__ Wn,
+                                           const double* __restrict__ H, const StepArgs& a, const TileCls& tc)
 {
     if (!(a.ctl->t < a.t_end)) return;     // the run has reached its end time
     const int nb = tc.tiles_x * tc.tiles_y;
@@ -829,6 +836,23 @@ __global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const float* __rest
                                     s, a.wn_prev ? a.ctl->cls_from : 0x7fffffff, go, Amax, a.ctl))
         return;
     block_max_atomic<NT>(Amax, &a.ctl->maxA_next);
+}
+
+template <int TX, int TY, int NT, bool BUF>
+__global__ void __launch_bounds__(NT) k_fused_bare(Consts k, const float* __restrict__ W, float* __restrict__ Wn,
+                                                   const double* __restrict__ H, StepArgs a, TileCls tc)
+{
+    fused_bare<TX, TY, NT, BUF>(k, W, Wn, H, a, tc);
+}
+
+// The same with 64-bit addresses, for states beyond state_buf_fits.
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) LH_BARE_SGPR_CAP k_fused_bare_a64(Consts k, const float* __restrict__ W,
+                                                                        float* __restrict__ Wn,
+                                                                        const double* __restrict__ H, StepArgs a,
+                                                                        TileCls tc)
+{
+    fused_bare<TX, TY, NT, false>(k, W, Wn, H, a, tc);
 }
 
 // General tiles of a step, then the step's close.  Each general class tile
@@ -1042,7 +1066,7 @@ __global__ void __launch_bounds__(NT) k_batch(const BatchScen* __restrict__ sd, 
 // scenario walked by its own group of gps workgroups.  Classes as in the
 // single-simulation step, per scenario.
 template <int TX, int TY, int NT, bool BUF>
-__global__ void __launch_bounds__(NT) k_batch_bare(const BatchScen* __restrict__ sd, const unsigned* __restrict__ tab)
+__device__ __forceinline__ void batch_bare(const BatchScen* __restrict__ sd, const unsigned* __restrict__ tab)
 {
     const unsigned e = tab[blockIdx.x];
     const BatchScen& d = sd[e >> 20];
@@ -1061,6 +1085,20 @@ __global__ void __launch_bounds__(NT) k_batch_bare(const BatchScen* __restrict__
                                     GaugeOut{nullptr, 0, nullptr}, Amax, ctl))
         return;
     block_max_atomic<NT>(Amax, &ctl->maxA_next);
+}
+
+template <int TX, int TY, int NT, bool BUF>
+__global__ void __launch_bounds__(NT) k_batch_bare(const BatchScen* __restrict__ sd, const unsigned* __restrict__ tab)
+{
+    batch_bare<TX, TY, NT, BUF>(sd, tab);
+}
+
+// The same with 64-bit addresses, for a batch with a state beyond state_buf_fits.
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) LH_BARE_SGPR_CAP k_batch_bare_a64(const BatchScen* __restrict__ sd,
+                                                                        const unsigned* __restrict__ tab)
+{
+    batch_bare<TX, TY, NT, false>(sd, tab);
 }
 
 // General sub-tiles of every scenario as one item space: item i of the
@@ -1279,7 +1317,7 @@ hipError_t f32c_step_fused(const Consts& k, const float* W, float* Wn, const dou
             hipLaunchKernelGGL((f32::k_fused_bare<f32::FTX, f32::FTY, f32::FNT, true>), dim3((unsigned)nt),
                                dim3(f32::FNT), 0, s, k, W, Wn, H, a, tc);
         else
-            hipLaunchKernelGGL((f32::k_fused_bare<f32::FTX, f32::FTY, f32::FNT, false>), dim3((unsigned)nt),
+            hipLaunchKernelGGL((f32::k_fused_bare_a64<f32::FTX, f32::FTY, f32::FNT>), dim3((unsigned)nt),
                                dim3(f32::FNT), 0, s, k, W, Wn, H, a, tc);
         hipLaunchKernelGGL((f32::k_fused_gen<f32::GSTX, f32::FTY, f32::GNT>), dim3(f32::GGRID), dim3(f32::GNT), 0, s,
                            k, W, Wn, H, a, tc);
@@ -1332,7 +1370,7 @@ hipError_t f32c_batch_step(int tile, const BatchScen* sd, int nscen, const unsig
             hipLaunchKernelGGL((f32::k_batch_bare<f32::FTX, f32::FTY, f32::FNT, true>), dim3(nblocks), dim3(f32::FNT),
                                0, s, sd, tab);
         else
-            hipLaunchKernelGGL((f32::k_batch_bare<f32::FTX, f32::FTY, f32::FNT, false>), dim3(nblocks),
+            hipLaunchKernelGGL((f32::k_batch_bare_a64<f32::FTX, f32::FTY, f32::FNT>), dim3(nblocks),
                                dim3(f32::FNT), 0, s, sd, tab);
         hipLaunchKernelGGL((f32::k_batch_gen<f32::GSTX, f32::FTY, f32::GNT>), dim3(f32::GGRID), dim3(f32::GNT), 0, s,
                            sd, nscen);

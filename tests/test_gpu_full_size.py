@@ -10,7 +10,8 @@ that hold at any size:
     (the mass fluxes of an edge are exact negatives of each other): relative
     drift <= 1e-12 (FP64) and <= 1e-9 (FP32C, compensated state) over 200 steps;
   * the FP32C step split by tile class (sediment-free and general launches)
-    equals every tile through the general tile function, bit for bit;
+    equals every tile through the general tile function, bit for bit, also
+    at 9728 x 9728, beyond 2^32 bytes of state (64-bit addressing);
   * precision comparison (the 2048^2 workload at its own 10 deg and with the
     slides flowing at 2 deg, 1000 steps, gauges near the slides): FP32C
     against FP64 on the gauge series (relative L2 of the signal eta - sea
@@ -96,7 +97,9 @@ def test_full_size_mass_conserved_with_walls(precision, bar):
     assert np.abs(g[4]).max() > 0
 
 
-@pytest.mark.parametrize("n,steps", [(2048, 120), (4096, 40)])
+# 9728 x 9728 (94.6 M cells) lies above 2^32 / 48 cells, where the
+# sediment-free kernel takes 64-bit addresses instead of buffer offsets
+@pytest.mark.parametrize("n,steps", [(2048, 120), (4096, 40), (9728, 6)])
 def test_full_size_split_step_equals_all_tiles(n, steps):
     sc, W, H, prm = workload(n, L.FP32C, angle=2.0)
     gauges = S.gauge_cells(sc, 16, seed=16)
