@@ -248,6 +248,14 @@ static void* state_buf(lhysea_sim* s, int b)
     return s->p.precision == LHYSEA_FP64 ? (void*)s->b64.W[b] : (void*)s->b32.W[b];
 }
 
+static void set_state_buf(lhysea_sim* s, int b, void* p)
+{
+    if (s->p.precision == LHYSEA_FP64)
+        s->b64.W[b] = (double*)p;
+    else
+        s->b32.W[b] = (float*)p;
+}
+
 static int launch_steps(lhysea_sim* s, int nsteps, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
                         double t_end = INFINITY)
 {
@@ -713,6 +721,9 @@ int lhysea_time_steps(lhysea_sim* s, int32_t nsteps, double* total_ms, double* k
 struct lhysea_adj_work {
     lhysea_sim* s = nullptr;
     char* snap = nullptr;  // snapshots of the state (or every state, store_all)
+    std::vector<char*> slot; // the buffer each snapshot slot holds (stores of the reverse sweep swap buffers)
+    void* own[2] = {nullptr, nullptr};  // the handle's state buffers, given back when the workspace goes
+    bool swap_store = false; // reverse sweep: a store hands the working buffer to the slot
     size_t state_bytes = 0;
     void* P = nullptr;     // partial sums [3][2][N]
     void* ACC = nullptr;   // [6][N]
@@ -746,6 +757,10 @@ void adj_free(lhysea_sim* s, lhysea_adj_work* a)
 {
     if (!a) return;
     hipStreamSynchronize(s->stream);
+    if (a->own[0]) {
+        set_state_buf(s, 0, a->own[0]);
+        set_state_buf(s, 1, a->own[1]);
+    }
     hipFree(a->snap);
     hipFree(a->P);
     hipFree(a->ACC);
@@ -792,13 +807,25 @@ hipError_t hook_advance(void* c, int k0, int n)
     return launch_steps(s, n) == LHYSEA_OK ? hipSuccess : hipErrorUnknown;
 }
 
+// In the reverse sweep a store of the working state copies nothing either:
+// the working buffer becomes the slot's and the slot's old buffer the
+// working one (the schedule restores the slot next, and the advance after a
+// restore rebuilds the tile classes and writes every plane of its first two
+// steps, so the old contents are never read).  The forward pass copies: it
+// steps on from the stored state.
 hipError_t hook_store(void* c, int slot)
 {
     lhysea_adj_work* a = (lhysea_adj_work*)c;
-    char* dst = a->snap + (size_t)slot * a->state_bytes;
-    const void* from = a->src ? (const void*)a->src : state_ptr(a->s);
+    lhysea_sim* s = a->s;
+    char* dst = a->slot[slot];
+    const void* from = a->src ? (const void*)a->src : state_ptr(s);
     if (from == dst) return hipSuccess;
-    return hipMemcpyAsync(dst, from, a->state_bytes, hipMemcpyDeviceToDevice, a->s->stream);
+    if (a->swap_store && !a->src) {
+        a->slot[slot] = (char*)state_ptr(s);
+        set_state_buf(s, s->cur, dst);
+        return hipSuccess;
+    }
+    return hipMemcpyAsync(dst, from, a->state_bytes, hipMemcpyDeviceToDevice, s->stream);
 }
 
 // A restore copies nothing: the next advance or reverse step reads the
@@ -806,7 +833,7 @@ hipError_t hook_store(void* c, int slot)
 hipError_t hook_restore(void* c, int slot)
 {
     lhysea_adj_work* a = (lhysea_adj_work*)c;
-    a->src = a->snap + (size_t)slot * a->state_bytes;
+    a->src = a->slot[slot];
     return hipSuccess;
 }
 
@@ -894,6 +921,10 @@ int lhysea_adjoint_forward(lhysea_sim* s, int32_t nsteps, int32_t nsnap, const i
         hipMalloc(&a->gmu, a->ngcomp * a->ngmu * sizeof(double)) != hipSuccess ||
         hipMalloc(&a->G, (size_t)nsteps * (ng > 0 ? ng : 1) * sizeof(double)) != hipSuccess)
         return done(fail(LHYSEA_ERR_GPU_MEMORY, "adjoint workspace allocation failed"));
+    a->slot.resize(nslots);
+    for (size_t i = 0; i < nslots; i++) a->slot[i] = a->snap + i * a->state_bytes;
+    a->own[0] = state_buf(s, 0);
+    a->own[1] = state_buf(s, 1);
     ADJCHK(hipEventCreate(&a->e0));
     ADJCHK(hipEventCreate(&a->e1));
     ADJCHK(hipEventCreate(&a->e2));
@@ -991,6 +1022,7 @@ static int adjoint_sweep(lhysea_sim* s, const double* seeds, double* grad_W0, do
         }
         a->W = nullptr;
     } else {
+        a->swap_store = true;
         he = run_treeverse(h, a->nsnap, nsteps);
     }
     if (he != hipSuccess) return done(hip_fail(he, "adjoint sweep"));
